@@ -1,0 +1,152 @@
+"""Throughput benchmark of the MI355X null-geodesic tracer.
+
+Workload (one step): four 2048x2048 Kerr frames, spins a in {0, 0.5, 0.9,
+0.998}, M = 1, observer at r = 1000 M, inclination 60 deg, vertical FOV 20 deg,
+DP45 rtol 1e-10 / atol 1e-12, capture at r_+ + 1e-6, escape at r = 1000 M
+outgoing, terminal disk z = 0 for r in [r_isco, 20 M]. Ray states are
+generated on the device and results stay in HBM (rt_trace_camera_resident).
+
+Prints one JSON line. value = rays/s over the timed steps. roofline: the
+tracing kernel's algorithmic FP64 FLOP per launch (right-hand-side
+evaluations x FLOP_RHS + attempted steps x FLOP_ATTEMPT, op-counted on the CPU
+oracle) over its HIP-event duration, against the MI355X FP64 vector peak.
+cpu_baseline: the CPU oracle (a clean-room C++ port, FP64) on every k-th row
+of the same frames, all host threads.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "relativity-simulator_amd"))
+
+import numpy as np  # noqa: E402
+
+import rtsim  # noqa: E402
+
+SPINS = (0.0, 0.5, 0.9, 0.998)
+SIDE = 2048
+INCL_DEG = 60.0
+FOV_DEG = 20.0
+R_OBS = 1000.0
+# Algorithmic FLOP per right-hand-side evaluation and per attempted DP45 step
+# (stage sums, error norm and controller over the 7 evolving components), from
+# the operation-counting instantiation of the CPU oracle; tests check that the
+# oracle still counts these values.
+FLOP_RHS = 178
+FLOP_ATTEMPT = 432
+PEAK_FP64_TFLOPS = 78.6
+
+
+def frame_requests(side: int, precision: int):
+    reqs = []
+    for a in SPINS:
+        cam = rtsim.default_camera(R_OBS, math.radians(INCL_DEG), 0.0, math.radians(FOV_DEG), side, side)
+        reqs.append((cam, rtsim.kerr_metric(1.0, a), rtsim.default_integrator(precision=precision),
+                     rtsim.default_termination()))
+    return reqs
+
+
+def cpu_baseline(side: int, stride: int, threads: int):
+    """Time the CPU oracle on rows 0, stride, 2*stride, ... of each frame."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib
+
+    native = os.path.join(REPO, "oracle", "liboracle_native.so")
+    if os.path.exists(native):
+        oracle_lib.ORACLE_PATH = native
+    rows = (side + stride - 1) // stride
+    n = 0
+    t0 = time.perf_counter()
+    for cam, met, ig, tm in frame_requests(side, rtsim.RT_PRECISION_FP64):
+        oracle_lib.trace_camera_rows(cam, met, ig, tm, 0, stride, rows, threads=threads)
+        n += rows * side
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th row of the four {side}x{side} frames ({n} rays, {dt:.1f} s), "
+                      f"oracle/{os.path.basename(oracle_lib.ORACLE_PATH)}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=SIDE)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cpu-stride", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this tracer runs on one GPU; --gpus {args.gpus} is not supported "
+              "(rays shard with zero exchange, see DESIGN.md)", file=sys.stderr)
+        return 2
+    prec = rtsim.RT_PRECISION_MIXED if args.precision == "mixed" else rtsim.RT_PRECISION_FP64
+    reqs = frame_requests(args.side, prec)
+    ctx = rtsim.Context(0)
+    print(f"bench: {len(reqs)} frames of {args.side}^2, precision {args.precision}, "
+          f"warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        for r in reqs:
+            ctx.trace_camera_resident(*r)
+        print(f"bench: warmup step {w + 1} done", file=sys.stderr, flush=True)
+    kernel_ms = 0.0
+    flops = 0.0
+    rays = 0
+    launches = 0
+    acc = rej = rhs = 0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        for r in reqs:
+            n = ctx.trace_camera_resident(*r)
+            st = ctx.stats()
+            kernel_ms += st.kernel_ms
+            flops += st.rhs_evals * FLOP_RHS + (st.accepted_steps + st.rejected_steps) * FLOP_ATTEMPT
+            rays += n
+            launches += 1
+            acc += st.accepted_steps
+            rej += st.rejected_steps
+            rhs += st.rhs_evals
+        print(f"bench: step {s + 1} done", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    value = rays / wall
+    avg_kernel_s = kernel_ms / launches / 1e3
+    achieved = flops / launches / avg_kernel_s / 1e12
+    out = {
+        "metric": "rays/s Kerr 2048^2 (4 spins, i=60 deg) DP45 rtol 1e-10",
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == rtsim.RT_PRECISION_FP64 else "f64+f32",
+        "data": "synthetic (camera rays generated on device)",
+        "config": {"workload": f"kerr_{args.side}sq_4spins_i60_robs1000_fov20", "side": args.side,
+                   "spins": list(SPINS), "precision": args.precision, "rays_per_step": rays // args.steps,
+                   "mean_accepted_steps": acc / rays, "mean_rejected_steps": rej / rays,
+                   "rhs_evals_per_ray": rhs / rays, "kernel_ms_per_frame": kernel_ms / launches},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        print(f"bench: cpu baseline on {threads} threads", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.side, args.cpu_stride, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

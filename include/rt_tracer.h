@@ -1,0 +1,243 @@
+/*
+ * rt_tracer.h — C ABI of the MI355X null-geodesic tracer (librtsim.so).
+ *
+ * One backward null geodesic per ray in Kerr / Kerr–Schild spacetime,
+ * integrated with adaptive Dormand–Prince 5(4) on the Kerr–Schild
+ * Hamiltonian, on one AMD Instinct MI355X (gfx950).
+ *
+ * Plain C: every type is a POD, every buffer is a pointer plus a count,
+ * no entry point throws, every entry point returns rt_status.
+ *
+ * Which reference interface each entry point replaces (paths relative to
+ * the Relativity-Simulator tree):
+ *   rt_trace_camera      SoftwareComputeEngine::dispatch_fp64
+ *                        (include/relativistic/render/software_compute_engine.hpp:5546-5587)
+ *                        and VulkanComputeExecutor::dispatch_and_readback
+ *                        (include/relativistic/render/vulkan_compute_executor.hpp:1399)
+ *   rt_trace_rays        per-ray tracing from an explicit (x, p):
+ *                        GeodesicRayProbe::trace
+ *                        (include/relativistic/optics/geodesic_ray_probe.hpp:452)
+ *   rt_trace_batch       many frames in one dispatch: the tile/band loop of
+ *                        TileRenderer::render (src/distributed/tile_renderer.cpp:133-182)
+ *                        and VulkanComputeExecutor banding (vulkan_compute_executor.hpp:1394-1465)
+ *   rt_server_*          the windowed worker of GeodesicComputePipeline
+ *                        (include/relativistic/render/geodesic_compute_pipeline.hpp:529-728):
+ *                        requests, cancellation, completion polling
+ */
+#ifndef RT_TRACER_H
+#define RT_TRACER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_ABI_VERSION 1u
+#define RT_MAX_SOURCES 16u
+
+/* ------------------------------------------------------------------ status */
+typedef enum rt_status {
+    RT_OK = 0,
+    RT_ERR_INVALID_ARG = 1, /* null pointer, zero size, out-of-range parameter */
+    RT_ERR_OOM = 2,         /* device or host allocation failed */
+    RT_ERR_DEVICE = 3,      /* HIP runtime error (message in rt_last_error) */
+    RT_ERR_TIMEOUT = 4,     /* rt_server_wait deadline passed */
+    RT_ERR_CANCELLED = 5,   /* request cancelled before completion */
+    RT_ERR_NO_DEVICE = 6,   /* no gfx950 device visible */
+    RT_ERR_BUSY = 7,        /* admission refused: footprint exceeds the budget */
+    RT_ERR_NOT_FOUND = 8    /* unknown request id */
+} rt_status;
+
+/* --------------------------------------------- per-ray class and flag bits
+ * The class bits are the reference's PixelFlags values
+ * (include/relativistic/render/gpu_types.hpp:1123-1130). */
+#define RT_CLS_CAPTURED 1u       /* HORIZON_ABSORBED: r <= r_+ + horizon_eps     */
+#define RT_CLS_ESCAPED 2u        /* CELESTIAL_HIT: r >= escape_radius, outgoing  */
+#define RT_CLS_DISK 4u           /* ACCRETION_DISK_HIT: z = 0 within the disk    */
+#define RT_CLS_MASK 7u
+#define RT_FLAG_BUDGET (1u << 8)     /* max_steps attempts used up            */
+#define RT_FLAG_MIN_STEP (1u << 9)   /* |h| fell below h_min                  */
+#define RT_FLAG_NONFINITE (1u << 10) /* state became non-finite               */
+#define RT_FLAG_PROJECTED (1u << 11) /* null-constraint projection fired once */
+#define RT_FLAG_FP32_STEPS (1u << 12)/* mixed precision: some steps ran FP32  */
+#define RT_FLAG_INVALID (1u << 15)   /* input ray rejected (not traced)       */
+
+/* ------------------------------------------------------------ parameters */
+typedef enum rt_precision {
+    RT_PRECISION_FP64 = 0,  /* every operation in FP64                          */
+    RT_PRECISION_MIXED = 1  /* FP32 right-hand side beyond mixed_switch_radius;
+                               FP64 state, error test, projection, root finding */
+} rt_precision;
+
+typedef struct rt_source {
+    /* One Kerr–Schild source of a superposition. Source 0 is the central hole
+     * (reference: build_multi_source_data, software_compute_engine.hpp:2738-2798). */
+    double position[3];   /* world Cartesian centre                               */
+    double mass;          /* m > 0                                                */
+    double spin;          /* a/m in [-0.9999, 0.9999] (dimensionless, like
+                             GpuBodyData::spin_parameter)                          */
+    double axis[3];       /* spin axis (normalised by the library)                */
+    double disk_inner;    /* <= 0: the source's prograde ISCO                     */
+    double disk_outer;    /* <= 0: 20 m;  < 0 on both: disk disabled            */
+    uint32_t disk_enabled;
+    uint32_t reserved;
+} rt_source;
+
+typedef struct rt_metric {
+    double mass;          /* M of the central hole (clamped to >= 1e-4)           */
+    double spin;          /* a in length units, clamped to [-0.999 M, 0.999 M]    */
+    uint32_t n_sources;   /* 0 or 1: single Kerr (mass, spin); 2..16: superposition
+                             described by src[0..n_sources-1], src[0] central     */
+    uint32_t reserved;
+    rt_source src[RT_MAX_SOURCES];
+} rt_metric;
+
+typedef struct rt_integrator {
+    double rtol;          /* DP45 relative tolerance (1e-10)                      */
+    double atol;          /* DP45 absolute tolerance (1e-12)                      */
+    double h0;            /* initial |h|; 0: the reference render-step heuristic  */
+    double h_min;         /* |h| below this ends the ray with RT_FLAG_MIN_STEP    */
+    double h_max;         /* cap on |h|                                           */
+    double invariant_tol; /* null-constraint projection threshold (1e-12)         */
+    double mixed_switch_radius; /* MIXED: FP64 RHS inside this radius; the library
+                                   raises it to at least 3 r_+                    */
+    uint32_t max_steps;   /* cap on attempted (accepted + rejected) steps         */
+    uint32_t precision;   /* rt_precision                                         */
+} rt_integrator;
+
+#define RT_TERM_NO_G_CLAMP 1u /* do not clamp disk g to [0.05, 4]                */
+
+typedef struct rt_termination {
+    double horizon_eps;   /* captured when r <= r_+ + horizon_eps (1e-6)           */
+    double escape_radius; /* escaped when r >= escape_radius while outgoing (1000) */
+    double disk_inner;    /* <= 0: prograde ISCO of (M, a)                        */
+    double disk_outer;    /* equatorial disk outer radius (20 M)                  */
+    uint32_t disk_enabled;
+    uint32_t flags;       /* RT_TERM_* bits                                        */
+} rt_termination;
+
+typedef struct rt_camera {
+    /* Pinhole camera, as GpuCameraPushConstants (render/gpu_types.hpp:966-1109):
+     * observer_position (t, r, theta, phi) and the spatial parts of
+     * tetrad_e1 (forward), tetrad_e2 (right), tetrad_e3 (up), world Cartesian. */
+    double observer[4];
+    double forward[3];
+    double right[3];
+    double up[3];
+    double fov_y;         /* vertical full field of view, radians                 */
+    uint32_t width;
+    uint32_t height;
+    uint32_t row_offset;  /* first traced row (dispatch_row_offset)               */
+    uint32_t row_count;   /* traced rows; 0 = all (dispatch_row_count)            */
+} rt_camera;
+
+typedef struct rt_results {
+    /* Caller-owned host arrays, SoA, one entry per traced ray (row-major pixel
+     * order for cameras). Any pointer may be NULL to skip that output. */
+    uint32_t* cls;        /* class bits | flag bits                               */
+    double* x;            /* 4*n: t[n] | x[n] | y[n] | z[n]  (Kerr–Schild)        */
+    double* p;            /* 4*n: p_t[n] | p_x[n] | p_y[n] | p_z[n] (covariant)   */
+    double* bl;           /* 3*n: r[n] | theta[n] | phi[n] (spheroidal, as
+                             boyer_lindquist_from_kerr_schild; phi in (-pi, pi])  */
+    double* lambda;       /* affine length sum |h| up to the end point            */
+    double* g;            /* disk: redshift_at; escaped: 1/E; captured: 0         */
+    uint32_t* steps;      /* accepted DP45 steps                                  */
+    uint32_t* rejected;   /* rejected DP45 steps                                  */
+} rt_results;
+
+typedef struct rt_stats {
+    double kernel_ms;          /* device time of the last trace (HIP events)      */
+    double total_ms;           /* host wall time of the last call                 */
+    uint64_t rays;             /* rays traced by the last call                    */
+    uint64_t accepted_steps;   /* summed over rays                                */
+    uint64_t rejected_steps;
+    uint64_t rhs_evals;        /* right-hand-side evaluations actually executed   */
+    uint64_t fp32_steps;       /* MIXED: steps that ran the FP32 right-hand side  */
+    uint32_t block_size;       /* launch shape used                               */
+    uint32_t grid_size;
+} rt_stats;
+
+typedef struct rt_request {
+    /* One camera request for batched tracing / serving. */
+    rt_camera camera;
+    rt_metric metric;
+    rt_integrator integrator;
+    rt_termination termination;
+} rt_request;
+
+typedef struct rt_context rt_context;
+typedef struct rt_server rt_server;
+
+/* ------------------------------------------------------------- defaults */
+/* Fill with the documented defaults (rtol 1e-10, atol 1e-12, ...). */
+void rt_default_integrator(rt_integrator* out);
+void rt_default_termination(rt_termination* out);
+/* Observer at (r, theta, phi) looking at the hole: forward = -e_r,
+ * right = e_phi, up = -e_theta. */
+void rt_default_camera(rt_camera* out, double r_obs, double theta_obs, double phi_obs,
+                       double fov_y, uint32_t width, uint32_t height);
+/* Kerr metric with mass M and spin a (length units). */
+void rt_kerr_metric(rt_metric* out, double mass, double spin);
+
+/* ------------------------------------------------------------- lifetime */
+uint32_t rt_abi_version(void);
+const char* rt_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* rt_last_error(void);
+
+rt_status rt_context_create(int device, rt_context** out);
+rt_status rt_context_destroy(rt_context* ctx);
+rt_status rt_get_stats(const rt_context* ctx, rt_stats* out);
+
+/* ------------------------------------------------------------ one frame */
+/* Trace every pixel of a camera; results are copied into host arrays. */
+rt_status rt_trace_camera(rt_context* ctx, const rt_camera* cam, const rt_metric* metric,
+                          const rt_integrator* integ, const rt_termination* term,
+                          rt_results* out);
+
+/* Trace n rays from explicit initial states (x^mu, p_mu) given as SoA host
+ * arrays of 4*n doubles each (t|x|y|z and p_t|p_x|p_y|p_z). */
+rt_status rt_trace_rays(rt_context* ctx, const double* x_soa, const double* p_soa, size_t n,
+                        const rt_metric* metric, const rt_integrator* integ,
+                        const rt_termination* term, rt_results* out);
+
+/* Device-resident variant of rt_trace_camera: results stay in device
+ * buffers owned by the context (valid until the next trace call on it);
+ * returns after the kernels finished. *n_rays receives the ray count. */
+rt_status rt_trace_camera_resident(rt_context* ctx, const rt_camera* cam, const rt_metric* metric,
+                                   const rt_integrator* integ, const rt_termination* term,
+                                   size_t* n_rays);
+/* Copy the device-resident results of the last resident trace to the host. */
+rt_status rt_fetch_results(rt_context* ctx, rt_results* out);
+
+/* ------------------------------------------------------- batched frames */
+/* Trace n_req camera requests of different sizes and tolerances in one
+ * persistent kernel fed by a device work queue. outs[i] receives request i. */
+rt_status rt_trace_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, rt_results* outs);
+
+/* ------------------------------------------------------------- serving */
+typedef struct rt_server_config {
+    uint64_t hbm_budget_bytes;  /* admission budget for ray state + results       */
+    uint32_t max_inflight;      /* requests resident at once                       */
+    uint32_t slots;             /* persistent lanes' work-queue capacity (rays)    */
+} rt_server_config;
+
+rt_status rt_server_create(int device, const rt_server_config* cfg, rt_server** out);
+rt_status rt_server_destroy(rt_server* srv);
+/* Admit a request (copied). Refused with RT_ERR_BUSY if its footprint can
+ * never fit the budget; otherwise queued until memory frees up. */
+rt_status rt_server_submit(rt_server* srv, const rt_request* req, uint64_t* id);
+/* 1 in *done when request id has finished (results ready for rt_server_take). */
+rt_status rt_server_poll(rt_server* srv, uint64_t id, int* done);
+/* Block until done or timeout_ms (< 0: no limit). */
+rt_status rt_server_wait(rt_server* srv, uint64_t id, int64_t timeout_ms);
+/* Copy the finished results into host arrays and release the request. */
+rt_status rt_server_take(rt_server* srv, uint64_t id, rt_results* out);
+rt_status rt_server_cancel(rt_server* srv, uint64_t id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RT_TRACER_H */

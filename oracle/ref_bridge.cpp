@@ -1,0 +1,142 @@
+// ref_bridge.cpp — extern "C" shims over the reference's own headers.
+//
+// TEST INFRASTRUCTURE ONLY. Compiled by oracle/Makefile against the
+// Relativity-Simulator headers where they lie (include path given on the
+// command line), into oracle/_ref/libref.so. Used by tests/ and by
+// tests/golden/make_golden.py to pin the clean-room oracle to the reference's
+// own functions; nothing here is reference code, only calls into it.
+#include <cmath>
+#include <cstdint>
+
+#include "relativistic/core/kerr_schild_null_integrator.hpp"
+#include "relativistic/integrators/rk45_adaptive.hpp"
+#include "relativistic/metrics/kerr_schild.hpp"
+#include "relativistic/optics/disk_thermal_profile.hpp"
+#include "relativistic/render/accretion_disk/accretion_disk_model.hpp"
+
+using namespace Relativistic;
+
+extern "C" {
+
+// core/kerr_schild_null_integrator.hpp:30-44
+void ref_ks_field(const double x[3], double M, double a, double out[5]) {
+    const auto f = Core::kerr_schild_field<double>({x[0], x[1], x[2]}, M, a);
+    out[0] = f.r;
+    out[1] = f.h;
+    out[2] = f.k[0];
+    out[3] = f.k[1];
+    out[4] = f.k[2];
+}
+
+// core/kerr_schild_null_integrator.hpp:71-114 (p_t = -1)
+void ref_ks_rhs(const double x[3], const double p[3], double M, double a, double dx[3], double dp[3]) {
+    Core::KerrSchildNullState<double> s;
+    s.position = {x[0], x[1], x[2]};
+    s.momentum = {p[0], p[1], p[2]};
+    const auto d = Core::kerr_schild_null_derivatives(s, M, a);
+    for (int i = 0; i < 3; ++i) {
+        dx[i] = d.position[i];
+        dp[i] = d.momentum[i];
+    }
+}
+
+// core/kerr_schild_null_integrator.hpp:116-139
+void ref_ks_rk4_step(double x[3], double p[3], double h, double M, double a) {
+    Core::KerrSchildNullState<double> s;
+    s.position = {x[0], x[1], x[2]};
+    s.momentum = {p[0], p[1], p[2]};
+    Core::step_kerr_schild_null_rk4(s, h, M, a);
+    for (int i = 0; i < 3; ++i) {
+        x[i] = s.position[i];
+        p[i] = s.momentum[i];
+    }
+}
+
+// core/kerr_schild_null_integrator.hpp:141-194
+void ref_initial_momentum(const double x[3], double M, double a, const double outward[3], double p[3]) {
+    const auto q = Core::kerr_schild_initial_momentum<double>({x[0], x[1], x[2]}, M, a,
+                                                             {outward[0], outward[1], outward[2]});
+    p[0] = q[0];
+    p[1] = q[1];
+    p[2] = q[2];
+}
+
+// core/kerr_schild_null_integrator.hpp:46-54
+void ref_ks_from_bl(double r, double th, double ph, double a, double out[3]) {
+    const auto q = Core::kerr_schild_position_from_boyer_lindquist<double>(r, th, ph, a);
+    out[0] = q[0];
+    out[1] = q[1];
+    out[2] = q[2];
+}
+
+// core/kerr_schild_null_integrator.hpp:56-69
+void ref_bl_from_ks(const double x[3], double a, double prev_phi, double out[3]) {
+    const auto q = Core::boyer_lindquist_from_kerr_schild<double>({x[0], x[1], x[2]}, a, prev_phi);
+    out[0] = q.r;
+    out[1] = q.theta;
+    out[2] = q.phi;
+}
+
+// optics/disk_thermal_profile.hpp:10-20 (same formula as the renderer's
+// private kerr_isco_radius, render/software_compute_engine.hpp:1052-1061)
+double ref_isco(double M, double a) { return Optics::DiskThermalProfile::kerr_isco_radius(M, a); }
+
+// render/accretion_disk/accretion_disk_model.hpp:205-225
+double ref_redshift(double M, double a, double r, double xi) {
+    return Render::AccretionDiskModel::redshift_at(M, a, r, xi);
+}
+
+// g_{mu nu} and g^{mu nu} of metrics/kerr_schild.hpp:141-189 (row-major 4x4).
+void ref_ks_metric(const double x4[4], double M, double a, double g[16], double ginv[16]) {
+    const Metrics::KerrSchildMetric<double> m(M, a);
+    const Core::FourVector<double> x(x4[0], x4[1], x4[2], x4[3]);
+    const auto G = m.metric_tensor(x);
+    const auto Gi = m.inverse_metric(x);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            g[i * 4 + j] = G(i, j);
+            ginv[i * 4 + j] = Gi(i, j);
+        }
+}
+
+// Integrate a null geodesic with the reference's Christoffel-based adaptive
+// DP45 (integrators/rk45_adaptive.hpp:158-313) on metrics/kerr_schild.hpp, from
+// (x0, u0) forward to affine parameter L exactly (the last step is shortened).
+// Returns 0 on success, 1 if the integrator gave up.
+int ref_rk45_integrate(const double x0[4], const double u0[4], double M, double a, double L, double rtol,
+                       double atol, double h0, double x_out[4], double u_out[4], uint64_t* acc, uint64_t* rej) {
+    const Metrics::KerrSchildMetric<double> m(M, a);
+    Integrators::RK45Config<double> cfg;
+    cfg.rtol = rtol;
+    cfg.atol = atol;
+    cfg.initial_step = h0;
+    cfg.max_step = 1e13;
+    Integrators::RK45AdaptiveIntegrator<Metrics::KerrSchildMetric<double>> integ(m, Integrators::GeodesicType::Null,
+                                                                               cfg);
+    Integrators::GeodesicState<double> s;
+    s.x = Core::FourVector<double>(x0[0], x0[1], x0[2], x0[3]);
+    s.u = Core::FourVector<double>(u0[0], u0[1], u0[2], u0[3]);
+    double lam = 0.0, dt = h0;
+    int guard = 0;
+    while (lam < L && guard++ < 10000000) {
+        double want = dt;
+        const bool last = (lam + want >= L);
+        if (last) want = L - lam;
+        double trial = want;
+        const auto used = integ.step(s, trial);
+        if (!used) return 1;
+        lam += *used;
+        // keep the controller's proposal unless this was the shortened last step
+        dt = last ? dt : trial;
+        if (last && *used == want) break;
+    }
+    for (int i = 0; i < 4; ++i) {
+        x_out[i] = s.x(i);
+        u_out[i] = s.u(i);
+    }
+    *acc = integ.statistics().accepted_steps;
+    *rej = integ.statistics().rejected_steps;
+    return 0;
+}
+
+}  // extern "C"

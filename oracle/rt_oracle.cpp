@@ -1,0 +1,950 @@
+// rt_oracle.cpp — FP64 CPU oracle: ray tracer, frame driver and C API.
+//
+// TEST INFRASTRUCTURE ONLY (see rt_oracle.hpp). Loaded by tests/ through
+// ctypes, by __graft_entry__.smoke() as the checker and by bench.py as the
+// CPU baseline; the product library never links it.
+//
+// Tracing semantics (the contract the HIP kernels also implement):
+//  * camera -> initial state restates dispatch_fp64_scalar's pixel loop and
+//    trace_kerr_schild_photon's set-up (render/software_compute_engine.hpp:3843-3938,
+//    1634-1652); multi-source set-up restates trace_multi_source
+//    (software_compute_engine.hpp:3054-3066) and ms_initial_momentum (:2919-2964);
+//  * stepping is adaptive DP45 (integrators/rk45_adaptive.hpp:196-311) on the
+//    Hamiltonian right-hand side, first-same-as-last when the state is not
+//    projected, with the null-constraint projection of rk45_adaptive.hpp:91-139
+//    restated for p_t;
+//  * events are located inside an accepted step on the DP45 dense output
+//    with an Illinois root finder: horizon r = r_+ + eps, equatorial disk z = 0
+//    with r in [r_in, r_out], escape r = r_esc while outgoing; the earliest wins;
+//  * exhausted budget / step underflow classify as captured when r < 3 r_s, as
+//    escaped otherwise (software_compute_engine.hpp:1767-1769).
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "rt_oracle.hpp"
+
+namespace rto {
+
+// Diagnostic counters (summed over all rays traced since the last reset).
+std::atomic<uint64_t> g_rhs_evals{0}, g_projections{0}, g_accepted{0}, g_rejected{0};
+
+struct Source {
+    double c[3];
+    double m, a, rh;
+    double fx[3], fy[3], ax[3];
+    double disk_in, disk_out;
+    bool disk_on;
+};
+
+struct Spacetime {
+    bool multi = false;
+    int n = 1;
+    double M = 1, a = 0, rh = 2;      // central hole (single-Kerr path)
+    double disk_in = 6, disk_out = 20, disk_spin = 0;
+    bool disk_on = true, clamp_g = true;
+    double eps_h = 1e-6, r_esc = 1000;
+    Source s[RT_MAX_SOURCES];
+};
+
+struct Integ {
+    double rtol, atol, h0, h_min, h_max, inv_tol;
+    uint32_t max_steps;
+};
+
+static double horizon_radius(double m, double a) { return m + sqrt(max(m * m - a * a, 0.0)); }
+
+// Restates the parameter set-up of dispatch_fp64_scalar (software_compute_engine.hpp:3801-3809)
+// and build_multi_source_data (:2738-2798).
+static Spacetime make_spacetime(const rt_metric* mt, const rt_termination* tm) {
+    Spacetime st;
+    st.eps_h = tm->horizon_eps;
+    st.r_esc = tm->escape_radius;
+    st.clamp_g = (tm->flags & RT_TERM_NO_G_CLAMP) == 0;
+    if (mt->n_sources >= 2) {
+        st.multi = true;
+        st.n = (int)std::min<uint32_t>(mt->n_sources, RT_MAX_SOURCES);
+        for (int i = 0; i < st.n; ++i) {
+            const rt_source& b = mt->src[i];
+            Source& s = st.s[i];
+            const double mb = (i == 0) ? max(b.mass, 1e-4) : max(b.mass, 1e-9);
+            const double ab = (i == 0) ? std::clamp(b.spin * mb, -0.999 * mb, 0.999 * mb)
+                                       : std::clamp(b.spin, -0.9999, 0.9999) * mb;
+            s.m = mb;
+            s.a = ab;
+            s.rh = horizon_radius(mb, ab);
+            for (int j = 0; j < 3; ++j) s.c[j] = b.position[j];
+            const double al = sqrt(b.axis[0] * b.axis[0] + b.axis[1] * b.axis[1] + b.axis[2] * b.axis[2]);
+            double ax[3] = {0, 0, 1};
+            if (al > 1e-12) {
+                ax[0] = b.axis[0] / al;
+                ax[1] = b.axis[1] / al;
+                ax[2] = b.axis[2] / al;
+            }
+            const double helper[3] = {abs(ax[2]) < 0.9 ? 0.0 : 1.0, 0.0, abs(ax[2]) < 0.9 ? 1.0 : 0.0};
+            double fx[3] = {helper[1] * ax[2] - helper[2] * ax[1], helper[2] * ax[0] - helper[0] * ax[2],
+                            helper[0] * ax[1] - helper[1] * ax[0]};
+            const double fl = sqrt(fx[0] * fx[0] + fx[1] * fx[1] + fx[2] * fx[2]);
+            fx[0] /= fl;
+            fx[1] /= fl;
+            fx[2] /= fl;
+            const double fy[3] = {ax[1] * fx[2] - ax[2] * fx[1], ax[2] * fx[0] - ax[0] * fx[2],
+                                  ax[0] * fx[1] - ax[1] * fx[0]};
+            // The central source keeps the world frame exactly, as build_multi_source_data does.
+            const bool world = (i == 0 && al > 1e-12 && ax[0] == 0.0 && ax[1] == 0.0 && ax[2] > 0.0);
+            for (int j = 0; j < 3; ++j) {
+                s.ax[j] = ax[j];
+                s.fx[j] = world ? (j == 0 ? 1.0 : 0.0) : fx[j];
+                s.fy[j] = world ? (j == 1 ? 1.0 : 0.0) : fy[j];
+            }
+            s.disk_on = b.disk_enabled != 0;
+            const double isco = kerr_isco(mb, ab);
+            const double din = (b.disk_inner > 0) ? b.disk_inner : isco;
+            s.disk_in = din;
+            s.disk_out = max((b.disk_outer > 0) ? b.disk_outer : 20.0 * mb, din * 1.05);
+        }
+        st.M = st.s[0].m;
+        st.a = st.s[0].a;
+        st.rh = st.s[0].rh;
+        st.disk_on = false;
+        return st;
+    }
+    st.M = max(mt->mass, 1e-4);
+    st.a = std::clamp(mt->spin, -0.999 * st.M, 0.999 * st.M);
+    st.rh = horizon_radius(st.M, st.a);
+    st.disk_spin = std::clamp(st.a, -0.999 * st.M, 0.999 * st.M);
+    st.disk_on = tm->disk_enabled != 0;
+    const double isco = kerr_isco(st.M, st.a);
+    st.disk_in = (tm->disk_inner > 0) ? tm->disk_inner : isco;
+    st.disk_out = max((tm->disk_outer > 0) ? tm->disk_outer : 20.0 * st.M, st.disk_in * 1.05);
+    return st;
+}
+
+static Integ make_integ(const rt_integrator* ig) {
+    Integ o;
+    o.rtol = ig->rtol;
+    o.atol = ig->atol;
+    o.h0 = ig->h0;
+    o.h_min = ig->h_min;
+    o.h_max = ig->h_max;
+    o.inv_tol = ig->invariant_tol;
+    o.max_steps = ig->max_steps;
+    return o;
+}
+
+// ---------------------------------------------------- multi-source pieces
+static inline void to_local(const Source& s, const double d[3], double l[3]) {
+    l[0] = d[0] * s.fx[0] + d[1] * s.fx[1] + d[2] * s.fx[2];
+    l[1] = d[0] * s.fy[0] + d[1] * s.fy[1] + d[2] * s.fy[2];
+    l[2] = d[0] * s.ax[0] + d[1] * s.ax[1] + d[2] * s.ax[2];
+}
+static inline void to_world(const Source& s, const double l[3], double d[3]) {
+    d[0] = l[0] * s.fx[0] + l[1] * s.fy[0] + l[2] * s.ax[0];
+    d[1] = l[0] * s.fx[1] + l[1] * s.fy[1] + l[2] * s.ax[1];
+    d[2] = l[0] * s.fx[2] + l[1] * s.fy[2] + l[2] * s.ax[2];
+}
+
+// Restates ms_rhs (software_compute_engine.hpp:2857-2870): sum of the rotated
+// per-source KS terms on top of the flat part, with the carried p_t.
+static void ms_rhs(const Spacetime& st, const double y[8], double d[8]) {
+    d[0] = -y[4];
+    d[1] = y[5];
+    d[2] = y[6];
+    d[3] = y[7];
+    d[4] = 0.0;
+    d[5] = d[6] = d[7] = 0.0;
+    for (int i = 0; i < st.n; ++i) {
+        const Source& s = st.s[i];
+        const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
+        double lx[3], lp[3];
+        to_local(s, dx, lx);
+        to_local(s, y + 5, lp);
+        double two_hs, corr[3], dp[3], wc[3], wp[3];
+        ks_terms(lx[0], lx[1], lx[2], y[4], lp[0], lp[1], lp[2], s.m, s.a, two_hs, corr, dp);
+        to_world(s, corr, wc);
+        to_world(s, dp, wp);
+        d[0] += two_hs;
+        d[1] += wc[0];
+        d[2] += wc[1];
+        d[3] += wc[2];
+        d[5] += wp[0];
+        d[6] += wp[1];
+        d[7] += wp[2];
+    }
+}
+
+static void rhs(const Spacetime& st, const double y[8], double d[8]) {
+    if (st.multi)
+        ms_rhs(st, y, d);
+    else
+        ks_rhs(y, st.M, st.a, d);
+}
+
+// Coefficients of the null constraint as a quadratic in p_t:
+//   g^{mu nu} p p = A p_t^2 + B p_t + C0
+static void constraint_quadratic(const Spacetime& st, const double y[8], double& A, double& B, double& C0) {
+    const double P2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
+    double sum_h = 0.0, sum_hq = 0.0, sum_hq2 = 0.0;
+    auto add = [&](double H, double q) {
+        sum_h += H;
+        sum_hq += H * q;
+        sum_hq2 += H * q * q;
+    };
+    if (!st.multi) {
+        const KSField<double> f = ks_field(y[1], y[2], y[3], st.M, st.a);
+        add(f.H, f.k0 * y[5] + f.k1 * y[6] + f.k2 * y[7]);
+    } else {
+        for (int i = 0; i < st.n; ++i) {
+            const Source& s = st.s[i];
+            const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
+            double lx[3], lp[3];
+            to_local(s, dx, lx);
+            to_local(s, y + 5, lp);
+            const KSField<double> f = ks_field(lx[0], lx[1], lx[2], s.m, s.a);
+            add(f.H, f.k0 * lp[0] + f.k1 * lp[1] + f.k2 * lp[2]);
+        }
+    }
+    A = -(1.0 + 2.0 * sum_h);
+    B = 4.0 * sum_hq;
+    C0 = P2 - 2.0 * sum_hq2;
+}
+
+double constraint(const Spacetime& st, const double y[8]) {
+    double A, B, C0;
+    constraint_quadratic(st, y, A, B, C0);
+    return (A * y[4] + B) * y[4] + C0;
+}
+
+// Null-constraint projection, restating project_invariant (rk45_adaptive.hpp:91-139)
+// for the Hamiltonian state: when |g^{mu nu} p p| >= tol, p_t is replaced by the
+// root of the quadratic closest to its current value. Returns true if it fired.
+static bool project(const Spacetime& st, double y[8], double tol) {
+    double A, B, C0;
+    constraint_quadratic(st, y, A, B, C0);
+    const double res = (A * y[4] + B) * y[4] + C0;
+    // Tolerance relative to the momentum scale: near r_+ the covariant components
+    // of a backward ray grow like 1/(r - r_+), and an absolute 1e-12 on a sum of
+    // terms of size |p|^2 would sit below the rounding of those terms.
+    const double scale = max(1.0, y[4] * y[4] + y[5] * y[5] + y[6] * y[6] + y[7] * y[7]);
+    if (!(abs(res) >= tol * scale)) return false;
+    if (abs(A) > 1e-14) {
+        const double disc = B * B - 4.0 * A * C0;
+        if (disc >= 0.0) {
+            const double sd = sqrt(disc);
+            const double r1 = (-B + sd) / (2.0 * A);
+            const double r2 = (-B - sd) / (2.0 * A);
+            y[4] = (abs(r1 - y[4]) <= abs(r2 - y[4])) ? r1 : r2;
+            return true;
+        }
+        return false;
+    }
+    if (abs(B) > 1e-14) {
+        y[4] = -C0 / B;
+        return true;
+    }
+    return false;
+}
+
+// Reference render step heuristic (software_compute_engine.hpp:1677-1683 with the
+// GpuCameraPushConstants defaults of render/gpu_types.hpp:981-1034) used as the
+// initial |h| when the caller gives none.
+static double initial_step(const Spacetime& st, const double y[8]) {
+    const double kappa = 0.5, hmin = 0.004, hmax = 3.5, sff = 7.8;
+    double scale = 1e300, blend = 1.0;
+    auto one = [&](double r, double rh, double m) {
+        scale = min(scale, sqrt(r * max(r - rh, 0.02 * m)));
+        blend = min(blend, std::clamp((r - 20.0 * rh) / (80.0 * rh), 0.0, 1.0));
+    };
+    if (!st.multi) {
+        one(ks_radius(y[1], y[2], y[3], st.a), st.rh, st.M);
+    } else {
+        for (int i = 0; i < st.n; ++i) {
+            const Source& s = st.s[i];
+            const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
+            double l[3];
+            to_local(s, dx, l);
+            one(ks_radius(l[0], l[1], l[2], s.a), s.rh, s.m);
+        }
+    }
+    const double far = 1.0 + (sff - 1.0) * blend;
+    return std::clamp(kappa * scale * far, hmin, hmax * sff);
+}
+
+// ------------------------------------------------------------------ rays
+struct RayOut {
+    uint32_t cls = 0;
+    double y[8] = {0};
+    double bl[3] = {0};
+    double lambda = 0, g = 0;
+    uint32_t acc = 0, rej = 0;
+};
+
+static void finish_bl(const Spacetime& st, RayOut& o) {
+    if (!st.multi) {
+        bl_from_ks(o.y + 1, st.a, o.bl);
+    } else {
+        const Source& s = st.s[0];
+        const double dx[3] = {o.y[1] - s.c[0], o.y[2] - s.c[1], o.y[3] - s.c[2]};
+        double l[3];
+        to_local(s, dx, l);
+        bl_from_ks(l, s.a, o.bl);
+    }
+}
+
+// Position of the ray relative to the escape/budget rules.
+static bool near_any_hole(const Spacetime& st, const double y[8]) {
+    if (!st.multi) return ks_radius(y[1], y[2], y[3], st.a) < 3.0 * (2.0 * st.M);
+    for (int i = 0; i < st.n; ++i) {
+        const Source& s = st.s[i];
+        const double dx = y[1] - s.c[0], dy = y[2] - s.c[1], dz = y[3] - s.c[2];
+        if (sqrt(dx * dx + dy * dy + dz * dz) < 6.0 * s.m) return true;
+    }
+    return false;
+}
+
+static double escape_radius_of(const Spacetime& st, const double* x) {
+    if (!st.multi) return ks_radius(x[0], x[1], x[2], st.a);
+    return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+}
+
+// Trace one ray from y_init (t, x, y, z, p_t, p_x, p_y, p_z). xi is L_z/E of
+// the ray for the single-hole disk redshift (reference: angular_momentum_z of the
+// initial state, software_compute_engine.hpp:1650).
+RayOut trace_ray(const Spacetime& st, const Integ& ig, const double y_init[8], double xi) {
+    RayOut o;
+    double y[8], y5[8], k[7][8];
+    std::memcpy(y, y_init, sizeof(y));
+    auto f = [&](const double* yy, double* dd) { rhs(st, yy, dd); };
+    f(y, k[0]);
+    double ah = (ig.h0 > 0) ? ig.h0 : initial_step(st, y);
+    ah = std::clamp(ah, ig.h_min, ig.h_max);
+    const double sgn = -1.0;  // backward in affine parameter
+    double lambda = 0.0;
+    uint32_t acc = 0, rej = 0, flags = 0;
+    uint32_t cls = 0;
+    bool done = false;
+    while (!done) {
+        if (acc + rej >= ig.max_steps || ah < ig.h_min) {
+            flags |= (acc + rej >= ig.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
+            cls = near_any_hole(st, y) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
+            std::memcpy(o.y, y, sizeof(y));
+            break;
+        }
+        const double h = sgn * ah;
+        const double err = dp45_attempt<double, 8>(f, y, h, ig.rtol, ig.atol, k, y5);
+        bool finite = std::isfinite(err);
+        for (int i = 0; i < 8; ++i) finite = finite && std::isfinite(y5[i]);
+        if (!finite) {
+            flags |= RT_FLAG_NONFINITE;
+            cls = RT_CLS_CAPTURED;
+            std::memcpy(o.y, y, sizeof(y));
+            break;
+        }
+        if (err <= 1.0) {
+            // ---- events inside the accepted step, earliest first
+            double best = 2.0;
+            uint32_t best_cls = 0;
+            double best_r = 0.0, best_g = 0.0;
+            auto dens = [&](double s, int i) { return dense_eval<8>(s, i, y, y5, h, k); };
+            auto dens_x = [&](double s, double out[3]) {
+                out[0] = dens(s, 1);
+                out[1] = dens(s, 2);
+                out[2] = dens(s, 3);
+            };
+            if (!st.multi) {
+                const double r0 = ks_radius(y[1], y[2], y[3], st.a);
+                const double r1 = ks_radius(y5[1], y5[2], y5[3], st.a);
+                const double r_hz = st.rh + st.eps_h;
+                if (r1 <= r_hz) {
+                    const double s = illinois_root(
+                        [&](double ss) {
+                            double x[3];
+                            dens_x(ss, x);
+                            return ks_radius(x[0], x[1], x[2], st.a) - r_hz;
+                        },
+                        r0 - r_hz, r1 - r_hz);
+                    if (s < best) {
+                        best = s;
+                        best_cls = RT_CLS_CAPTURED;
+                    }
+                }
+                const double z0 = y[3], z1 = y5[3];
+                if (st.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0))) {
+                    const double s = illinois_root([&](double ss) { return dens(ss, 3); }, z0, z1);
+                    double x[3];
+                    dens_x(s, x);
+                    const double rc = ks_radius(x[0], x[1], x[2], st.a);
+                    if (rc >= st.disk_in && rc <= st.disk_out && rc > st.rh * 1.02 && s < best) {
+                        best = s;
+                        best_cls = RT_CLS_DISK;
+                        best_r = rc;
+                        best_g = disk_redshift(st.M, st.disk_spin, rc, xi, st.clamp_g);
+                    }
+                }
+                if (r1 >= st.r_esc && sgn * (y5[1] * k[6][1] + y5[2] * k[6][2] + y5[3] * k[6][3]) > 0.0) {
+                    double s = 1.0;
+                    if (r0 < st.r_esc) {
+                        s = illinois_root(
+                            [&](double ss) {
+                                double x[3];
+                                dens_x(ss, x);
+                                return ks_radius(x[0], x[1], x[2], st.a) - st.r_esc;
+                            },
+                            r0 - st.r_esc, r1 - st.r_esc);
+                    }
+                    if (s < best) {
+                        best = s;
+                        best_cls = RT_CLS_ESCAPED;
+                    }
+                }
+            } else {
+                for (int i = 0; i < st.n; ++i) {
+                    const Source& src = st.s[i];
+                    auto local = [&](const double* x, double l[3]) {
+                        const double d[3] = {x[0] - src.c[0], x[1] - src.c[1], x[2] - src.c[2]};
+                        to_local(src, d, l);
+                    };
+                    double l0[3], l1[3];
+                    local(y + 1, l0);
+                    local(y5 + 1, l1);
+                    const double r0 = ks_radius(l0[0], l0[1], l0[2], src.a);
+                    const double r1 = ks_radius(l1[0], l1[1], l1[2], src.a);
+                    const double r_hz = src.rh + st.eps_h;
+                    if (r1 <= r_hz && r0 > r_hz) {
+                        const double s = illinois_root(
+                            [&](double ss) {
+                                double x[3], l[3];
+                                dens_x(ss, x);
+                                local(x, l);
+                                return ks_radius(l[0], l[1], l[2], src.a) - r_hz;
+                            },
+                            r0 - r_hz, r1 - r_hz);
+                        if (s < best) {
+                            best = s;
+                            best_cls = RT_CLS_CAPTURED;
+                        }
+                    }
+                    const double z0 = l0[2], z1 = l1[2];
+                    if (src.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0))) {
+                        const double s = illinois_root(
+                            [&](double ss) {
+                                double x[3], l[3];
+                                dens_x(ss, x);
+                                local(x, l);
+                                return l[2];
+                            },
+                            z0, z1);
+                        double x[3], l[3];
+                        dens_x(s, x);
+                        local(x, l);
+                        const double rc = ks_radius(l[0], l[1], l[2], src.a);
+                        if (rc >= src.disk_in && rc <= src.disk_out && rc > src.rh * 1.02 && s < best) {
+                            // xi = local L_z over |p|, as ms_disk_crossing (software_compute_engine.hpp:3021-3042).
+                            const double pw[3] = {dens(s, 5), dens(s, 6), dens(s, 7)};
+                            double pl[3];
+                            to_local(src, pw, pl);
+                            const double pm = sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+                            const double lz = (l[0] * pl[1] - l[1] * pl[0]) / max(pm, 1e-30);
+                            best = s;
+                            best_cls = RT_CLS_DISK;
+                            best_r = rc;
+                            best_g = disk_redshift(src.m, src.a, rc, lz, st.clamp_g);
+                        }
+                    }
+                }
+                const double R0 = escape_radius_of(st, y + 1), R1 = escape_radius_of(st, y5 + 1);
+                if (R1 >= st.r_esc && sgn * (y5[1] * k[6][1] + y5[2] * k[6][2] + y5[3] * k[6][3]) > 0.0) {
+                    double s = 1.0;
+                    if (R0 < st.r_esc) {
+                        s = illinois_root(
+                            [&](double ss) {
+                                double x[3];
+                                dens_x(ss, x);
+                                return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) - st.r_esc;
+                            },
+                            R0 - st.r_esc, R1 - st.r_esc);
+                    }
+                    if (s < best) {
+                        best = s;
+                        best_cls = RT_CLS_ESCAPED;
+                    }
+                }
+            }
+            ++acc;
+            if (best_cls != 0) {
+                if (best >= 1.0) {
+                    std::memcpy(o.y, y5, sizeof(y5));
+                } else {
+                    for (int i = 0; i < 8; ++i) o.y[i] = dens(best, i);
+                }
+                if (project(st, o.y, ig.inv_tol)) flags |= RT_FLAG_PROJECTED;
+                lambda += best * ah;
+                cls = best_cls;
+                if (cls == RT_CLS_DISK) {
+                    o.g = best_g;
+                    (void)best_r;
+                }
+                done = true;
+                break;
+            }
+            std::memcpy(y, y5, sizeof(y));
+            lambda += ah;
+            if (project(st, y, ig.inv_tol)) {
+                flags |= RT_FLAG_PROJECTED;
+                g_projections.fetch_add(1, std::memory_order_relaxed);
+                f(y, k[0]);
+            } else {
+                std::memcpy(k[0], k[6], sizeof(k[0]));
+            }
+            ah = dp45_next_step(ah, err, true, ig.h_min, ig.h_max);
+        } else {
+            ++rej;
+            ah = dp45_next_step(ah, err, false, ig.h_min, ig.h_max);
+        }
+    }
+    g_accepted.fetch_add(acc, std::memory_order_relaxed);
+    g_rejected.fetch_add(rej, std::memory_order_relaxed);
+    o.cls = cls | flags;
+    o.lambda = lambda;
+    o.acc = acc;
+    o.rej = rej;
+    if ((cls & RT_CLS_MASK) == RT_CLS_ESCAPED) o.g = 1.0 / (-o.y[4]);
+    if ((cls & RT_CLS_MASK) == RT_CLS_CAPTURED) o.g = 0.0;
+    finish_bl(st, o);
+    return o;
+}
+
+// -------------------------------------------------------------- camera
+// Pixel -> initial state, restating dispatch_fp64_scalar (software_compute_engine.hpp:3843-3938),
+// CameraProjector::compute_pinhole_ray (observer/camera_projections.hpp:27-41) and the
+// set-up of trace_kerr_schild_photon (:1634-1652) / trace_multi_source (:3054-3066).
+// Returns false when the observer sits inside a horizon (captured at once).
+bool camera_ray(const Spacetime& st, const rt_camera* cam, uint32_t px, uint32_t py, double y[8], double* xi) {
+    const double W = cam->width, Hh = cam->height;
+    const double aspect = W / Hh;
+    const double v = 1.0 - (static_cast<double>(py) + 0.5) / Hh * 2.0;
+    const double u = ((static_cast<double>(px) + 0.5) / W * 2.0 - 1.0) * aspect;
+    const double tan_half = std::tan(cam->fov_y * 0.5);
+    const double n1 = 1.0, n2 = v * tan_half, n3 = u * tan_half;
+    const double len = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
+    const double inv_len = (len > 0.0) ? (1.0 / len) : 1.0;
+    const double nl[3] = {n1 * inv_len, n2 * inv_len, n3 * inv_len};
+    const double d[3] = {nl[0] * cam->forward[0] + nl[2] * cam->right[0] + nl[1] * cam->up[0],
+                         nl[0] * cam->forward[1] + nl[2] * cam->right[1] + nl[1] * cam->up[1],
+                         nl[0] * cam->forward[2] + nl[2] * cam->right[2] + nl[1] * cam->up[2]};
+    const double r_obs = max(cam->observer[1], st.rh * 1.02);
+    const double th = std::clamp(cam->observer[2], 0.001, kPi - 0.001);
+    const double ph = cam->observer[3];
+    const double st_ = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
+    const double n_r = d[0] * (st_ * cp) + d[1] * (st_ * sp) + d[2] * ct;
+    const double n_th = d[0] * (ct * cp) + d[1] * (ct * sp) + d[2] * (-st_);
+    const double n_ph = d[0] * (-sp) + d[1] * cp + d[2] * 0.0;
+    const double outward[3] = {n_r * st_ * cp + n_th * ct * cp - n_ph * sp, n_r * st_ * sp + n_th * ct * sp + n_ph * cp,
+                               n_r * ct - n_th * st_};
+    y[0] = 0.0;
+    y[4] = -1.0;
+    if (!st.multi) {
+        ks_position_from_bl(r_obs, th, ph, st.a, y + 1);
+        ks_initial_momentum(y + 1, st.M, st.a, outward, y + 5);
+        *xi = y[1] * y[6] - y[2] * y[5];
+        return true;
+    }
+    // multi-source: flat spherical position, superposed-metric frame.
+    y[1] = r_obs * st_ * cp;
+    y[2] = r_obs * st_ * sp;
+    y[3] = r_obs * ct;
+    *xi = 0.0;
+    for (int i = 0; i < st.n; ++i) {
+        const Source& s = st.s[i];
+        const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
+        double l[3];
+        to_local(s, dx, l);
+        const double a2 = s.a * s.a;
+        const double diff = (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]) - a2;
+        const double r2 = max(0.5 * (diff + sqrt(diff * diff + 4.0 * a2 * l[2] * l[2])), 1e-24);
+        const double r = sqrt(r2);
+        if (r <= s.rh * 1.0001 || (r2 - 2.0 * s.m * r + a2) <= 0.0) return false;
+    }
+    // ms_initial_momentum (software_compute_engine.hpp:2919-2964)
+    double pot[RT_MAX_SOURCES], dir[RT_MAX_SOURCES][3];
+    double psum = 0.0;
+    for (int i = 0; i < st.n; ++i) {
+        const Source& s = st.s[i];
+        const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
+        double l[3];
+        to_local(s, dx, l);
+        const KSField<double> f = ks_field(l[0], l[1], l[2], s.m, s.a);
+        const double kl[3] = {f.k0, f.k1, f.k2};
+        pot[i] = f.H;
+        to_world(s, kl, dir[i]);
+        psum += f.H;
+    }
+    const int n = st.n;
+    auto mdot = [&](const double* A, const double* B) {
+        double v = -A[0] * B[0] + A[1] * B[1] + A[2] * B[2] + A[3] * B[3];
+        for (int i = 0; i < n; ++i) {
+            const double la = A[0] + dir[i][0] * A[1] + dir[i][1] * A[2] + dir[i][2] * A[3];
+            const double lb = B[0] + dir[i][0] * B[1] + dir[i][1] * B[2] + dir[i][2] * B[3];
+            v += 2.0 * pot[i] * la * lb;
+        }
+        return v;
+    };
+    const double ils = 1.0 / max(1.0 - 2.0 * psum, 1e-6);
+    const double u0 = sqrt(ils);
+    auto lift = [&](double vx, double vy, double vz, double out[4]) {
+        double shift = 0.0;
+        for (int i = 0; i < n; ++i) shift += pot[i] * (dir[i][0] * vx + dir[i][1] * vy + dir[i][2] * vz);
+        out[0] = 2.0 * shift * ils;
+        out[1] = vx;
+        out[2] = vy;
+        out[3] = vz;
+    };
+    double e1[4], e2[4], e3[4];
+    lift(1, 0, 0, e1);
+    lift(0, 1, 0, e2);
+    lift(0, 0, 1, e3);
+    const double n1_ = sqrt(max(mdot(e1, e1), 1e-30));
+    for (int j = 0; j < 4; ++j) e1[j] /= n1_;
+    const double e21 = mdot(e2, e1);
+    for (int j = 0; j < 4; ++j) e2[j] -= e21 * e1[j];
+    const double n2_ = sqrt(max(mdot(e2, e2), 1e-30));
+    for (int j = 0; j < 4; ++j) e2[j] /= n2_;
+    const double e31 = mdot(e3, e1), e32 = mdot(e3, e2);
+    for (int j = 0; j < 4; ++j) e3[j] -= e31 * e1[j] + e32 * e2[j];
+    const double n3_ = sqrt(max(mdot(e3, e3), 1e-30));
+    for (int j = 0; j < 4; ++j) e3[j] /= n3_;
+    const double vv[4] = {u0 - outward[0] * e1[0] - outward[1] * e2[0] - outward[2] * e3[0],
+                          -outward[0] * e1[1] - outward[1] * e2[1] - outward[2] * e3[1],
+                          -outward[0] * e1[2] - outward[1] * e2[2] - outward[2] * e3[2],
+                          -outward[0] * e1[3] - outward[1] * e2[3] - outward[2] * e3[3]};
+    double mom[3] = {vv[1], vv[2], vv[3]};
+    double tc = -vv[0];
+    for (int i = 0; i < n; ++i) {
+        const double lv = vv[0] + dir[i][0] * vv[1] + dir[i][1] * vv[2] + dir[i][2] * vv[3];
+        tc += 2.0 * pot[i] * lv;
+        mom[0] += 2.0 * pot[i] * lv * dir[i][0];
+        mom[1] += 2.0 * pot[i] * lv * dir[i][1];
+        mom[2] += 2.0 * pot[i] * lv * dir[i][2];
+    }
+    const double inv_pt = 1.0 / max(-tc, 1e-12);
+    y[5] = mom[0] * inv_pt;
+    y[6] = mom[1] * inv_pt;
+    y[7] = mom[2] * inv_pt;
+    return true;
+}
+
+static void store(rt_results* out, size_t n, size_t idx, const RayOut& o) {
+    if (out->cls) out->cls[idx] = o.cls;
+    if (out->x)
+        for (int c = 0; c < 4; ++c) out->x[c * n + idx] = o.y[c];
+    if (out->p)
+        for (int c = 0; c < 4; ++c) out->p[c * n + idx] = o.y[4 + c];
+    if (out->bl)
+        for (int c = 0; c < 3; ++c) out->bl[c * n + idx] = o.bl[c];
+    if (out->lambda) out->lambda[idx] = o.lambda;
+    if (out->g) out->g[idx] = o.g;
+    if (out->steps) out->steps[idx] = o.acc;
+    if (out->rejected) out->rejected[idx] = o.rej;
+}
+
+template <class Fn>
+static void parallel_rows(size_t rows, int threads, Fn&& fn) {
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const size_t r = next.fetch_add(1);
+            if (r >= rows) return;
+            fn(r);
+        }
+    };
+    if (threads == 1) {
+        worker();
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+}
+
+}  // namespace rto
+
+// ===================================================================== C API
+using namespace rto;
+
+extern "C" {
+
+int rto_trace_camera(const rt_camera* cam, const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm,
+                     rt_results* out, int threads) {
+    if (!cam || !mt || !ig || !tm || !out || cam->width == 0 || cam->height == 0) return RT_ERR_INVALID_ARG;
+    const Spacetime st = make_spacetime(mt, tm);
+    const Integ in = make_integ(ig);
+    const uint32_t first = std::min(cam->row_offset, cam->height);
+    const uint32_t last = cam->row_count ? std::min(first + cam->row_count, cam->height) : cam->height;
+    const size_t W = cam->width;
+    const size_t n = (size_t)(last - first) * W;
+    parallel_rows(last - first, threads, [&](size_t row) {
+        const uint32_t py = first + (uint32_t)row;
+        for (uint32_t px = 0; px < cam->width; ++px) {
+            double y[8], xi;
+            RayOut o;
+            if (camera_ray(st, cam, px, py, y, &xi)) {
+                o = trace_ray(st, in, y, xi);
+            } else {
+                std::memcpy(o.y, y, sizeof(y));
+                o.cls = RT_CLS_CAPTURED;
+                finish_bl(st, o);
+            }
+            store(out, n, row * W + px, o);
+        }
+    });
+    return RT_OK;
+}
+
+// Same as rto_trace_camera on a subset of rows: rows first + j*stride, j < count.
+// Output index is j*W + x (dense in the subset).
+int rto_trace_camera_rows(const rt_camera* cam, const rt_metric* mt, const rt_integrator* ig,
+                          const rt_termination* tm, uint32_t first, uint32_t stride, uint32_t count,
+                          rt_results* out, int threads) {
+    if (!cam || !mt || !ig || !tm || !out || stride == 0) return RT_ERR_INVALID_ARG;
+    const Spacetime st = make_spacetime(mt, tm);
+    const Integ in = make_integ(ig);
+    const size_t W = cam->width;
+    const size_t n = (size_t)count * W;
+    parallel_rows(count, threads, [&](size_t j) {
+        const uint32_t py = first + (uint32_t)j * stride;
+        for (uint32_t px = 0; px < cam->width; ++px) {
+            double y[8], xi;
+            RayOut o;
+            if (py < cam->height && camera_ray(st, cam, px, py, y, &xi)) {
+                o = trace_ray(st, in, y, xi);
+            } else {
+                o.cls = RT_CLS_CAPTURED;
+            }
+            store(out, n, j * W + px, o);
+        }
+    });
+    return RT_OK;
+}
+
+int rto_trace_rays(const double* x_soa, const double* p_soa, size_t n, const rt_metric* mt, const rt_integrator* ig,
+                   const rt_termination* tm, rt_results* out, int threads) {
+    if (!x_soa || !p_soa || !mt || !ig || !tm || !out) return RT_ERR_INVALID_ARG;
+    const Spacetime st = make_spacetime(mt, tm);
+    const Integ in = make_integ(ig);
+    const size_t chunk = 64;
+    parallel_rows((n + chunk - 1) / chunk, threads, [&](size_t c) {
+        for (size_t i = c * chunk; i < std::min(n, (c + 1) * chunk); ++i) {
+            double y[8];
+            for (int j = 0; j < 4; ++j) {
+                y[j] = x_soa[j * n + i];
+                y[4 + j] = p_soa[j * n + i];
+            }
+            const double xi = (y[1] * y[6] - y[2] * y[5]) / (-y[4]);
+            store(out, n, i, trace_ray(st, in, y, xi));
+        }
+    });
+    return RT_OK;
+}
+
+// Initial state of one pixel (ok = 0 if the observer is inside a horizon).
+int rto_camera_state(const rt_camera* cam, const rt_metric* mt, const rt_termination* tm, uint32_t px, uint32_t py,
+                     double y[8], double* xi) {
+    const Spacetime st = make_spacetime(mt, tm);
+    return camera_ray(st, cam, px, py, y, xi) ? 1 : 0;
+}
+
+// Diagnostic counters: accepted, rejected, projections; reset when reset != 0.
+void rto_counters(uint64_t out[3], int reset) {
+    out[0] = g_accepted.load();
+    out[1] = g_rejected.load();
+    out[2] = g_projections.load();
+    if (reset) {
+        g_accepted = 0;
+        g_rejected = 0;
+        g_projections = 0;
+    }
+}
+
+void rto_ks_field(const double x[3], double M, double a, double out[5]) {
+    const KSField<double> f = ks_field(x[0], x[1], x[2], M, a);
+    out[0] = f.r;
+    out[1] = f.H;
+    out[2] = f.k0;
+    out[3] = f.k1;
+    out[4] = f.k2;
+}
+
+void rto_rhs(const double y[8], const rt_metric* mt, double d[8]) {
+    rt_termination tm{};
+    tm.horizon_eps = 1e-6;
+    tm.escape_radius = 1000;
+    tm.disk_outer = 20;
+    const Spacetime st = make_spacetime(mt, &tm);
+    rhs(st, y, d);
+}
+
+double rto_constraint(const double y[8], const rt_metric* mt) {
+    rt_termination tm{};
+    tm.horizon_eps = 1e-6;
+    tm.escape_radius = 1000;
+    const Spacetime st = make_spacetime(mt, &tm);
+    return constraint(st, y);
+}
+
+int rto_project(double y[8], const rt_metric* mt, double tol) {
+    rt_termination tm{};
+    tm.horizon_eps = 1e-6;
+    tm.escape_radius = 1000;
+    const Spacetime st = make_spacetime(mt, &tm);
+    return project(st, y, tol) ? 1 : 0;
+}
+
+void rto_initial_momentum(const double x[3], double M, double a, const double outward[3], double p[3]) {
+    ks_initial_momentum(x, M, a, outward, p);
+}
+
+void rto_ks_from_bl(double r, double th, double ph, double a, double out[3]) { ks_position_from_bl(r, th, ph, a, out); }
+void rto_bl_from_ks(const double x[3], double a, double out[3]) { bl_from_ks(x, a, out); }
+double rto_isco(double M, double a) { return kerr_isco(M, a); }
+double rto_redshift(double M, double a, double r, double xi, int clamp) { return disk_redshift(M, a, r, xi, clamp != 0); }
+
+// One DP45 attempt on the geodesic system from y with step h; fills y5, the
+// seven stage derivatives k (7x8, row-major) and returns the error norm.
+double rto_dp45_attempt(const double y[8], double h, const rt_metric* mt, double rtol, double atol, double y5[8],
+                        double kout[56]) {
+    rt_termination tm{};
+    tm.horizon_eps = 1e-6;
+    tm.escape_radius = 1000;
+    const Spacetime st = make_spacetime(mt, &tm);
+    double k[7][8];
+    rhs(st, y, k[0]);
+    auto f = [&](const double* yy, double* dd) { rhs(st, yy, dd); };
+    const double err = dp45_attempt<double, 8>(f, y, h, rtol, atol, k, y5);
+    for (int s = 0; s < 7; ++s)
+        for (int i = 0; i < 8; ++i) kout[s * 8 + i] = k[s][i];
+    return err;
+}
+
+// Fixed-step DP45 on test problems with closed forms, for pinning the tableau
+// and the dense output: kind 0: y' = lam*y; kind 1: y' = 5 t^4 (as an autonomous
+// 2-system (t, y)); kind 2: harmonic oscillator (x' = v, v' = -x).
+// Integrates n_steps of size h from (t0, y0[0..1]); writes y_end[0..1], and the
+// dense value at fraction s of the LAST step in dense_out[0..1]; returns max err norm.
+double rto_test_ode(int kind, double lam, const double y0in[2], double t0, double h, int n_steps, double s,
+                    double y_end[2], double dense_out[2]) {
+    constexpr int N = 3;  // (t, y0, y1)
+    auto f = [&](const double* yy, double* dd) {
+        dd[0] = 1.0;
+        if (kind == 0) {
+            dd[1] = lam * yy[1];
+            dd[2] = lam * yy[2];
+        } else if (kind == 1) {
+            const double t = yy[0];
+            dd[1] = 5.0 * t * t * t * t;
+            dd[2] = 4.0 * t * t * t;
+        } else {
+            dd[1] = yy[2];
+            dd[2] = -yy[1];
+        }
+    };
+    double y[N] = {t0, y0in[0], y0in[1]}, y5[N], k[7][N];
+    double emax = 0.0;
+    for (int i = 0; i < n_steps; ++i) {
+        f(y, k[0]);
+        const double e = dp45_attempt<double, N>(f, y, h, 1e-300, 1e-300, k, y5);
+        emax = max(emax, e);
+        if (i == n_steps - 1) {
+            dense_out[0] = dense_eval<N>(s, 1, y, y5, h, k);
+            dense_out[1] = dense_eval<N>(s, 2, y, y5, h, k);
+        }
+        for (int j = 0; j < N; ++j) y[j] = y5[j];
+    }
+    y_end[0] = y[1];
+    y_end[1] = y[2];
+    return emax;
+}
+
+// Illinois root of a polynomial c0 + c1 s + c2 s^2 + c3 s^3 on [0,1].
+double rto_test_root(const double c[4]) {
+    auto g = [&](double s) { return ((c[3] * s + c[2]) * s + c[1]) * s + c[0]; };
+    return illinois_root(g, g(0.0), g(1.0));
+}
+
+// Trace a single ray given its initial state (for small-case tests).
+void rto_trace_one(const double y0[8], const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm,
+                   uint32_t* cls, double y_end[8], double bl[3], double* lambda, double* g, uint32_t* acc,
+                   uint32_t* rej) {
+    const Spacetime st = make_spacetime(mt, tm);
+    const Integ in = make_integ(ig);
+    const double xi = (y0[1] * y0[6] - y0[2] * y0[5]) / (-y0[4]);
+    const RayOut o = trace_ray(st, in, y0, xi);
+    *cls = o.cls;
+    for (int i = 0; i < 8; ++i) y_end[i] = o.y[i];
+    for (int i = 0; i < 3; ++i) bl[i] = o.bl[i];
+    *lambda = o.lambda;
+    *g = o.g;
+    *acc = o.acc;
+    *rej = o.rej;
+}
+
+}  // extern "C"
+
+// ======================================================= FLOP counting
+// Instantiates the templated RHS and DP45 attempt with a scalar that counts
+// + - * / and sqrt (1 each; pow counts 1), to fix the algorithmic FLOP
+// constants used for roofline reporting.
+namespace rto_count {
+static thread_local uint64_t g_ops = 0;
+struct Cnt {
+    double v;
+    Cnt() : v(0) {}
+    Cnt(double x) : v(x) {}
+};
+inline Cnt operator+(Cnt a, Cnt b) { ++g_ops; return Cnt(a.v + b.v); }
+inline Cnt operator-(Cnt a, Cnt b) { ++g_ops; return Cnt(a.v - b.v); }
+inline Cnt operator*(Cnt a, Cnt b) { ++g_ops; return Cnt(a.v * b.v); }
+inline Cnt operator/(Cnt a, Cnt b) { ++g_ops; return Cnt(a.v / b.v); }
+inline Cnt operator-(Cnt a) { return Cnt(-a.v); }
+inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
+inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
+inline bool operator<=(Cnt a, Cnt b) { return a.v <= b.v; }
+inline bool operator==(Cnt a, Cnt b) { return a.v == b.v; }
+inline Cnt sqrt(Cnt a) { ++g_ops; return Cnt(std::sqrt(a.v)); }
+inline Cnt abs(Cnt a) { return Cnt(std::fabs(a.v)); }
+inline Cnt pow(Cnt a, Cnt b) { ++g_ops; return Cnt(std::pow(a.v, b.v)); }
+inline Cnt max(Cnt a, Cnt b) { return a.v < b.v ? b : a; }
+inline Cnt min(Cnt a, Cnt b) { return b.v < a.v ? b : a; }
+}  // namespace rto_count
+
+namespace rto {
+using rto_count::Cnt;
+inline Cnt sqrt(Cnt a) { return rto_count::sqrt(a); }
+inline Cnt abs(Cnt a) { return rto_count::abs(a); }
+inline Cnt pow(Cnt a, Cnt b) { return rto_count::pow(a, b); }
+inline Cnt max(Cnt a, Cnt b) { return rto_count::max(a, b); }
+}  // namespace rto
+
+extern "C" void rto_flop_counts(double out[4]) {
+    using rto_count::Cnt;
+    using rto_count::g_ops;
+    Cnt y[8] = {0.0, 700.0, 200.0, 300.0, -1.0, -0.7, -0.2, -0.3}, d[8];
+    g_ops = 0;
+    rto::ks_rhs<Cnt>(y, Cnt(1.0), Cnt(0.9), d);
+    out[0] = static_cast<double>(g_ops);  // one right-hand-side evaluation
+    Cnt k[7][8], y5[8];
+    for (int i = 0; i < 8; ++i) k[0][i] = d[i];
+    uint64_t rhs_ops = 0;
+    auto f = [&](const Cnt* yy, Cnt* dd) {
+        const uint64_t before = g_ops;
+        rto::ks_rhs<Cnt>(yy, Cnt(1.0), Cnt(0.9), dd);
+        rhs_ops += g_ops - before;
+    };
+    g_ops = 0;
+    const Cnt err = rto::dp45_attempt<Cnt, 8>(f, y, Cnt(-20.0), Cnt(1e-10), Cnt(1e-12), k, y5);
+    (void)rto::dp45_next_step<Cnt>(Cnt(20.0), err, true, Cnt(1e-12), Cnt(1e13));
+    out[1] = static_cast<double>(g_ops - rhs_ops);  // stage sums + error norm + controller per attempt
+    out[2] = static_cast<double>(rhs_ops);          // six RHS evaluations inside an attempt
+    out[3] = 0.0;
+}

@@ -1,0 +1,374 @@
+"""Python binding of librtsim.so, the MI355X null-geodesic tracer.
+
+A thin ctypes mirror of include/rt_tracer.h: the same structs, the same entry
+points, results as numpy arrays. The C++/HIP library does all the work; this
+module only marshals arguments. It refuses to run when the library is missing
+(there is no fallback path).
+
+Reference interfaces mirrored (paths relative to the Relativity-Simulator tree):
+  Context.trace_camera   SoftwareComputeEngine::dispatch_fp64
+                         (include/relativistic/render/software_compute_engine.hpp:5546)
+  Context.trace_rays     GeodesicRayProbe::trace
+                         (include/relativistic/optics/geodesic_ray_probe.hpp:452)
+  Context.trace_batch    TileRenderer::render (src/distributed/tile_renderer.cpp:133)
+  Server                 GeodesicComputePipeline windowed worker
+                         (include/relativistic/render/geodesic_compute_pipeline.hpp:529-728)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "librtsim.so")
+
+RT_MAX_SOURCES = 16
+RT_CLS_CAPTURED = 1
+RT_CLS_ESCAPED = 2
+RT_CLS_DISK = 4
+RT_CLS_MASK = 7
+RT_FLAG_BUDGET = 1 << 8
+RT_FLAG_MIN_STEP = 1 << 9
+RT_FLAG_NONFINITE = 1 << 10
+RT_FLAG_PROJECTED = 1 << 11
+RT_FLAG_FP32_STEPS = 1 << 12
+RT_FLAG_INVALID = 1 << 15
+RT_PRECISION_FP64 = 0
+RT_PRECISION_MIXED = 1
+RT_TERM_NO_G_CLAMP = 1
+
+STATUS_NAMES = {
+    0: "RT_OK", 1: "RT_ERR_INVALID_ARG", 2: "RT_ERR_OOM", 3: "RT_ERR_DEVICE", 4: "RT_ERR_TIMEOUT",
+    5: "RT_ERR_CANCELLED", 6: "RT_ERR_NO_DEVICE", 7: "RT_ERR_BUSY", 8: "RT_ERR_NOT_FOUND",
+}
+
+
+class RtError(RuntimeError):
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+
+
+class rt_source(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("mass", C.c_double), ("spin", C.c_double),
+                ("axis", C.c_double * 3), ("disk_inner", C.c_double), ("disk_outer", C.c_double),
+                ("disk_enabled", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class rt_metric(C.Structure):
+    _fields_ = [("mass", C.c_double), ("spin", C.c_double), ("n_sources", C.c_uint32),
+                ("reserved", C.c_uint32), ("src", rt_source * RT_MAX_SOURCES)]
+
+
+class rt_integrator(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("h0", C.c_double), ("h_min", C.c_double),
+                ("h_max", C.c_double), ("invariant_tol", C.c_double), ("mixed_switch_radius", C.c_double),
+                ("max_steps", C.c_uint32), ("precision", C.c_uint32)]
+
+
+class rt_termination(C.Structure):
+    _fields_ = [("horizon_eps", C.c_double), ("escape_radius", C.c_double), ("disk_inner", C.c_double),
+                ("disk_outer", C.c_double), ("disk_enabled", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class rt_camera(C.Structure):
+    _fields_ = [("observer", C.c_double * 4), ("forward", C.c_double * 3), ("right", C.c_double * 3),
+                ("up", C.c_double * 3), ("fov_y", C.c_double), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("row_offset", C.c_uint32), ("row_count", C.c_uint32)]
+
+
+class rt_results(C.Structure):
+    _fields_ = [("cls", C.POINTER(C.c_uint32)), ("x", C.POINTER(C.c_double)), ("p", C.POINTER(C.c_double)),
+                ("bl", C.POINTER(C.c_double)), ("lam", C.POINTER(C.c_double)), ("g", C.POINTER(C.c_double)),
+                ("steps", C.POINTER(C.c_uint32)), ("rejected", C.POINTER(C.c_uint32))]
+
+
+class rt_stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64),
+                ("accepted_steps", C.c_uint64), ("rejected_steps", C.c_uint64), ("rhs_evals", C.c_uint64),
+                ("fp32_steps", C.c_uint64), ("block_size", C.c_uint32), ("grid_size", C.c_uint32)]
+
+
+class rt_request(C.Structure):
+    _fields_ = [("camera", rt_camera), ("metric", rt_metric), ("integrator", rt_integrator),
+                ("termination", rt_termination)]
+
+
+class rt_server_config(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32), ("slots", C.c_uint32)]
+
+
+# ------------------------------------------------------------ defaults
+def default_integrator(rtol: float = 1e-10, atol: float = 1e-12, precision: int = RT_PRECISION_FP64,
+                       max_steps: int = 20000, switch_radius: float = 0.0) -> rt_integrator:
+    """Same values as rt_default_integrator()."""
+    ig = rt_integrator()
+    ig.rtol, ig.atol, ig.h0, ig.h_min, ig.h_max = rtol, atol, 0.0, 1e-12, 1e13
+    ig.invariant_tol = 1e-12
+    ig.mixed_switch_radius = switch_radius
+    ig.max_steps = max_steps
+    ig.precision = precision
+    return ig
+
+
+def default_termination(escape_radius: float = 1000.0, disk_outer: float = 20.0, horizon_eps: float = 1e-6,
+                        disk_enabled: bool = True, flags: int = 0) -> rt_termination:
+    """Same values as rt_default_termination()."""
+    t = rt_termination()
+    t.horizon_eps, t.escape_radius, t.disk_inner, t.disk_outer = horizon_eps, escape_radius, 0.0, disk_outer
+    t.disk_enabled = 1 if disk_enabled else 0
+    t.flags = flags
+    return t
+
+
+def default_camera(r_obs: float, theta_obs: float, phi_obs: float, fov_y: float, width: int, height: int,
+                   row_offset: int = 0, row_count: int = 0) -> rt_camera:
+    """Observer looking at the hole: forward = -e_r, right = e_phi, up = -e_theta
+    (same as rt_default_camera())."""
+    c = rt_camera()
+    st, ct, sp, cp = math.sin(theta_obs), math.cos(theta_obs), math.sin(phi_obs), math.cos(phi_obs)
+    c.observer[:] = [0.0, r_obs, theta_obs, phi_obs]
+    c.forward[:] = [-st * cp, -st * sp, -ct]
+    c.right[:] = [-sp, cp, 0.0]
+    c.up[:] = [-ct * cp, -ct * sp, st]
+    c.fov_y = fov_y
+    c.width, c.height, c.row_offset, c.row_count = width, height, row_offset, row_count
+    return c
+
+
+def kerr_metric(mass: float, spin: float) -> rt_metric:
+    m = rt_metric()
+    m.mass, m.spin, m.n_sources = mass, spin, 0
+    return m
+
+
+def multi_metric(sources: Sequence[dict]) -> rt_metric:
+    """Superposition of 2..16 Kerr-Schild sources; sources[0] is the central
+    hole. Each dict: position (3), mass, spin (a/m), axis (3), disk_enabled,
+    disk_inner, disk_outer."""
+    if not 2 <= len(sources) <= RT_MAX_SOURCES:
+        raise ValueError("multi_metric takes 2..16 sources")
+    m = rt_metric()
+    m.mass = sources[0]["mass"]
+    m.spin = sources[0].get("spin", 0.0) * sources[0]["mass"]
+    m.n_sources = len(sources)
+    for i, s in enumerate(sources):
+        d = m.src[i]
+        d.position[:] = list(s.get("position", (0.0, 0.0, 0.0)))
+        d.mass = s["mass"]
+        d.spin = s.get("spin", 0.0)
+        d.axis[:] = list(s.get("axis", (0.0, 0.0, 1.0)))
+        d.disk_inner = s.get("disk_inner", 0.0)
+        d.disk_outer = s.get("disk_outer", 0.0)
+        d.disk_enabled = 1 if s.get("disk_enabled", True) else 0
+    return m
+
+
+# ------------------------------------------------------------ results
+class Results:
+    """Host SoA result arrays for n rays, pointed to by an rt_results."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.cls = np.zeros(n, np.uint32)
+        self.x = np.zeros((4, n), np.float64)
+        self.p = np.zeros((4, n), np.float64)
+        self.bl = np.zeros((3, n), np.float64)
+        self.lam = np.zeros(n, np.float64)
+        self.g = np.zeros(n, np.float64)
+        self.steps = np.zeros(n, np.uint32)
+        self.rejected = np.zeros(n, np.uint32)
+        self.struct = rt_results()
+        self.struct.cls = self.cls.ctypes.data_as(C.POINTER(C.c_uint32))
+        self.struct.x = self.x.ctypes.data_as(C.POINTER(C.c_double))
+        self.struct.p = self.p.ctypes.data_as(C.POINTER(C.c_double))
+        self.struct.bl = self.bl.ctypes.data_as(C.POINTER(C.c_double))
+        self.struct.lam = self.lam.ctypes.data_as(C.POINTER(C.c_double))
+        self.struct.g = self.g.ctypes.data_as(C.POINTER(C.c_double))
+        self.struct.steps = self.steps.ctypes.data_as(C.POINTER(C.c_uint32))
+        self.struct.rejected = self.rejected.ctypes.data_as(C.POINTER(C.c_uint32))
+
+    @property
+    def klass(self) -> np.ndarray:
+        return self.cls & RT_CLS_MASK
+
+    def as_dict(self) -> Dict[str, np.ndarray]:
+        return {"cls": self.cls, "x": self.x, "p": self.p, "bl": self.bl, "lam": self.lam, "g": self.g,
+                "steps": self.steps, "rejected": self.rejected}
+
+
+def n_rays_of(cam: rt_camera) -> int:
+    first = min(cam.row_offset, cam.height)
+    last = min(first + cam.row_count, cam.height) if cam.row_count else cam.height
+    return (last - first) * cam.width
+
+
+# ------------------------------------------------------------ library
+_LIB = None
+
+EXPORTS = [
+    "rt_default_integrator", "rt_default_termination", "rt_default_camera", "rt_kerr_metric",
+    "rt_abi_version", "rt_version", "rt_last_error", "rt_context_create", "rt_context_destroy",
+    "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
+    "rt_trace_batch", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
+    "rt_server_wait", "rt_server_take", "rt_server_cancel",
+]
+
+
+def load(path: Optional[str] = None) -> C.CDLL:
+    """Load librtsim.so (built by __graft_entry__.build()); raises if absent."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"librtsim.so not built at {p}; run __graft_entry__.build()")
+    lib = C.CDLL(p)
+    P = C.POINTER
+    lib.rt_version.restype = C.c_char_p
+    lib.rt_last_error.restype = C.c_char_p
+    lib.rt_abi_version.restype = C.c_uint32
+    lib.rt_context_create.argtypes = [C.c_int, P(C.c_void_p)]
+    lib.rt_context_destroy.argtypes = [C.c_void_p]
+    lib.rt_get_stats.argtypes = [C.c_void_p, P(rt_stats)]
+    lib.rt_trace_camera.argtypes = [C.c_void_p, P(rt_camera), P(rt_metric), P(rt_integrator),
+                                    P(rt_termination), P(rt_results)]
+    lib.rt_trace_rays.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_size_t, P(rt_metric),
+                                  P(rt_integrator), P(rt_termination), P(rt_results)]
+    lib.rt_trace_camera_resident.argtypes = [C.c_void_p, P(rt_camera), P(rt_metric), P(rt_integrator),
+                                             P(rt_termination), P(C.c_size_t)]
+    lib.rt_fetch_results.argtypes = [C.c_void_p, P(rt_results)]
+    lib.rt_trace_batch.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_results)]
+    lib.rt_default_integrator.argtypes = [P(rt_integrator)]
+    lib.rt_default_termination.argtypes = [P(rt_termination)]
+    lib.rt_default_camera.argtypes = [P(rt_camera), C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_uint32, C.c_uint32]
+    lib.rt_kerr_metric.argtypes = [P(rt_metric), C.c_double, C.c_double]
+    lib.rt_server_create.argtypes = [C.c_int, P(rt_server_config), P(C.c_void_p)]
+    lib.rt_server_destroy.argtypes = [C.c_void_p]
+    lib.rt_server_submit.argtypes = [C.c_void_p, P(rt_request), P(C.c_uint64)]
+    lib.rt_server_poll.argtypes = [C.c_void_p, C.c_uint64, P(C.c_int)]
+    lib.rt_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_int64]
+    lib.rt_server_take.argtypes = [C.c_void_p, C.c_uint64, P(rt_results)]
+    lib.rt_server_cancel.argtypes = [C.c_void_p, C.c_uint64]
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _check(lib, status: int) -> None:
+    if status != 0:
+        raise RtError(status, lib.rt_last_error().decode())
+
+
+class Context:
+    """One device context (device buffers, stream, kernel statistics)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        _check(self.lib, self.lib.rt_context_create(device, C.byref(self.h)))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.rt_context_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def stats(self) -> rt_stats:
+        s = rt_stats()
+        _check(self.lib, self.lib.rt_get_stats(self.h, C.byref(s)))
+        return s
+
+    def trace_camera(self, cam: rt_camera, metric: rt_metric, integ: rt_integrator,
+                     term: rt_termination) -> Results:
+        res = Results(n_rays_of(cam))
+        _check(self.lib, self.lib.rt_trace_camera(self.h, C.byref(cam), C.byref(metric), C.byref(integ),
+                                                  C.byref(term), C.byref(res.struct)))
+        return res
+
+    def trace_camera_resident(self, cam: rt_camera, metric: rt_metric, integ: rt_integrator,
+                              term: rt_termination) -> int:
+        n = C.c_size_t()
+        _check(self.lib, self.lib.rt_trace_camera_resident(self.h, C.byref(cam), C.byref(metric),
+                                                           C.byref(integ), C.byref(term), C.byref(n)))
+        return n.value
+
+    def fetch(self, n: int) -> Results:
+        res = Results(n)
+        _check(self.lib, self.lib.rt_fetch_results(self.h, C.byref(res.struct)))
+        return res
+
+    def trace_rays(self, x: np.ndarray, p: np.ndarray, metric: rt_metric, integ: rt_integrator,
+                   term: rt_termination) -> Results:
+        x = np.ascontiguousarray(x, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        if x.ndim != 2 or x.shape[0] != 4 or p.shape != x.shape:
+            raise ValueError("x and p must be (4, n) arrays")
+        n = x.shape[1]
+        res = Results(n)
+        _check(self.lib, self.lib.rt_trace_rays(self.h, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                                p.ctypes.data_as(C.POINTER(C.c_double)), n, C.byref(metric),
+                                                C.byref(integ), C.byref(term), C.byref(res.struct)))
+        return res
+
+    def trace_batch(self, reqs: Sequence[rt_request]) -> List[Results]:
+        arr = (rt_request * len(reqs))(*reqs)
+        outs = [Results(n_rays_of(r.camera)) for r in reqs]
+        oarr = (rt_results * len(reqs))(*[o.struct for o in outs])
+        _check(self.lib, self.lib.rt_trace_batch(self.h, arr, len(reqs), oarr))
+        return outs
+
+
+class Server:
+    """Continuous-batching request loop on one device."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, max_inflight: int = 64,
+                 slots: int = 0):
+        self.lib = load()
+        cfg = rt_server_config(hbm_budget_bytes, max_inflight, slots)
+        self.h = C.c_void_p()
+        _check(self.lib, self.lib.rt_server_create(device, C.byref(cfg), C.byref(self.h)))
+        self._sizes: Dict[int, int] = {}
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.rt_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, req: rt_request) -> int:
+        rid = C.c_uint64()
+        _check(self.lib, self.lib.rt_server_submit(self.h, C.byref(req), C.byref(rid)))
+        self._sizes[rid.value] = n_rays_of(req.camera)
+        return rid.value
+
+    def poll(self, rid: int) -> bool:
+        d = C.c_int()
+        _check(self.lib, self.lib.rt_server_poll(self.h, rid, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> None:
+        _check(self.lib, self.lib.rt_server_wait(self.h, rid, timeout_ms))
+
+    def take(self, rid: int) -> Results:
+        res = Results(self._sizes.pop(rid))
+        _check(self.lib, self.lib.rt_server_take(self.h, rid, C.byref(res.struct)))
+        return res
+
+    def cancel(self, rid: int) -> None:
+        _check(self.lib, self.lib.rt_server_cancel(self.h, rid))
