@@ -38,6 +38,42 @@ constexpr double d1 = -12715105075.0 / 11282082432.0, d3 = 87487479700.0 / 32700
 
 constexpr double kPi = 3.14159265358979323846;
 
+// ---------------------------------------------------- fast FP64 math
+// Reciprocal of a positive normal x: v_rcp_f64 (about 2^-29 relative) and two
+// Newton steps, within about 1 ulp (the correctly rounded division sequence
+// costs twice the instructions).
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// sqrt(x) and 1/sqrt(x) of a positive normal x: v_rsq_f64 and two Goldschmidt
+// iterations plus a final residual correction of the root.
+__device__ __forceinline__ double sqrt_rsqrt64(double x, double& rs) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, x);
+    rs = h + h;
+    return fma(d, h, g);
+}
+
+// sqrt of x >= 0 (0 allowed), same sequence.
+__device__ __forceinline__ double sqrt64(double x) {
+    double rs;
+    const double v = sqrt_rsqrt64(x, rs);
+    return (x > 0.0) ? v : 0.0;
+}
+
 struct Field {
     double r, H, k0, k1, k2;
 };
@@ -46,8 +82,9 @@ struct Field {
 __device__ __forceinline__ double ks_radius(double x, double y, double z, double a2) {
     const double z2 = z * z;
     const double diff = x * x + y * y + z2 - a2;
-    const double r2 = fmax(0.5 * (diff + sqrt(diff * diff + 4.0 * a2 * z2)), 1e-24);
-    return sqrt(r2);
+    const double r2 = fmax(0.5 * (diff + sqrt64(diff * diff + 4.0 * a2 * z2)), 1e-24);
+    double rs;
+    return sqrt_rsqrt64(r2, rs);
 }
 
 __device__ __forceinline__ Field ks_field(double x, double y, double z, double M, double a, double a2) {
@@ -68,72 +105,97 @@ __device__ __forceinline__ Field ks_field(double x, double y, double z, double M
 
 // Hamiltonian right-hand side of one source in its own frame:
 // two_hs = 2 H s, corr = -2 H s k, dp = s^2 grad H + 2 H s grad(k.p), s = k.p - p_t.
-__device__ __forceinline__ Field ks_terms(double X, double Y, double Z, double pt, double px, double py, double pz,
-                                          double M, double a, double a2, double& two_hs, double corr[3],
-                                          double dp[3]) {
-    const double z2 = Z * Z;
-    const double diff = X * X + Y * Y + z2 - a2;
-    const double r2 = fmax(0.5 * (diff + sqrt(diff * diff + 4.0 * a2 * z2)), 1e-24);
-    const double r = sqrt(r2);
-    const double r3 = r2 * r;
-    const double r4 = r2 * r2;
-    const double den = r4 + a2 * z2;
-    const double rpa = r2 + a2;
-    // three reciprocals replace the reference's eleven divisions
-    const double inv_den = 1.0 / den;
-    const double inv_rpa = 1.0 / rpa;
-    const double inv_r = 1.0 / r;
-    const double H = M * r3 * inv_den;
-    const double k0 = (r * X + a * Y) * inv_rpa;
-    const double k1 = (r * Y - a * X) * inv_rpa;
-    const double k2 = Z * inv_r;
-    const double s = k0 * px + k1 * py + k2 * pz - pt;
-    two_hs = 2.0 * H * s;
-    corr[0] = -(two_hs * k0);
-    corr[1] = -(two_hs * k1);
-    corr[2] = -(two_hs * k2);
+// Generic over the arithmetic type R (double for FP64 steps, float for the
+// far-field steps of the mixed-precision policy); inputs and outputs are FP64.
+template <class R>
+__device__ __forceinline__ R rcpR(R x);
+template <>
+__device__ __forceinline__ double rcpR<double>(double x) {
+    return rcp64(x);
+}
+template <>
+__device__ __forceinline__ float rcpR<float>(float x) {
+    return __builtin_amdgcn_rcpf(x);
+}
+template <class R>
+__device__ __forceinline__ R sqrt_rsqrtR(R x, R& rs);
+template <>
+__device__ __forceinline__ double sqrt_rsqrtR<double>(double x, double& rs) {
+    return sqrt_rsqrt64(x, rs);
+}
+template <>
+__device__ __forceinline__ float sqrt_rsqrtR<float>(float x, float& rs) {
+    rs = __builtin_amdgcn_rsqf(x);
+    return x * rs;
+}
+template <class R>
+__device__ __forceinline__ R sqrtR(R x) {
+    R rs;
+    const R v = sqrt_rsqrtR<R>(x, rs);
+    return (x > R(0)) ? v : R(0);
+}
 
-    const double inv_den2 = inv_den * inv_den;
-    const double inv_rpa2 = inv_rpa * inv_rpa;
-    const double inv_r2 = inv_r * inv_r;
-    const double r3d = r3 * inv_den;
-    const double dr0 = r3d * X, dr1 = r3d * Y, dr2 = r * rpa * Z * inv_den;
-    const double bracket = 3.0 * a2 * z2 - r4;
-    const double nx = r * X + a * Y;
-    const double ny = r * Y - a * X;
-    const double hb = M * r2 * bracket * inv_den2;  // dH = dr_i * hb (+ z term)
-    const double ss = s * s;
-    const double two_h_s = 2.0 * H * s;
-    const double two_r = 2.0 * r;
-    // grad(k.p) per component, written out (ex/ey/ez unit terms folded in)
-    const double kx0 = ((dr0 * X + r) * rpa - nx * two_r * dr0) * inv_rpa2;
-    const double ky0 = ((dr0 * Y - a) * rpa - ny * two_r * dr0) * inv_rpa2;
-    const double kz0 = (-Z * dr0) * inv_r2;
-    const double kx1 = ((dr1 * X + a) * rpa - nx * two_r * dr1) * inv_rpa2;
-    const double ky1 = ((dr1 * Y + r) * rpa - ny * two_r * dr1) * inv_rpa2;
-    const double kz1 = (-Z * dr1) * inv_r2;
-    const double kx2 = ((dr2 * X) * rpa - nx * two_r * dr2) * inv_rpa2;
-    const double ky2 = ((dr2 * Y) * rpa - ny * two_r * dr2) * inv_rpa2;
-    const double kz2 = (r - Z * dr2) * inv_r2;
-    const double dh0 = dr0 * hb;
-    const double dh1 = dr1 * hb;
-    const double dh2 = dr2 * hb - 2.0 * M * a2 * r3 * Z * inv_den2;
-    dp[0] = dh0 * ss + two_h_s * (kx0 * px + ky0 * py + kz0 * pz);
-    dp[1] = dh1 * ss + two_h_s * (kx1 * px + ky1 * py + kz1 * pz);
-    dp[2] = dh2 * ss + two_h_s * (kx2 * px + ky2 * py + kz2 * pz);
+// The gradient of k.p is evaluated in closed form instead of through the
+// nine-entry Jacobian of k: with u = x p_x + y p_y, w = y p_x - x p_y,
+// rho = r^2 + a^2, q1 = (r u + a w)/rho and q2 = z p_z / r (so k.p = q1 + q2),
+//   grad(k.p) = A + C grad r,  A = ((r p_x - a p_y)/rho, (r p_y + a p_x)/rho, p_z/r),
+//   C = (u - 2 r q1)/rho - q2/r,
+// and grad H = hb grad r + (0, 0, -2 M a^2 r^3 z / den^2). Same function as
+// the reference's expression, about half the operations.
+template <class R>
+__device__ __forceinline__ Field ks_terms(double Xd, double Yd, double Zd, double ptd, double pxd, double pyd,
+                                          double pzd, double Md, double ad, double a2d, double& two_hs,
+                                          double corr[3], double dp[3]) {
+    const R X = R(Xd), Y = R(Yd), Z = R(Zd), pt = R(ptd), px = R(pxd), py = R(pyd), pz = R(pzd);
+    const R M = R(Md), a = R(ad), a2 = R(a2d);
+    const R z2 = Z * Z;
+    const R diff = X * X + Y * Y + z2 - a2;
+    const R r2 = fmax(R(0.5) * (diff + sqrtR<R>(diff * diff + R(4) * a2 * z2)), R(1e-24));
+    R inv_r;
+    const R r = sqrt_rsqrtR<R>(r2, inv_r);
+    const R r3 = r2 * r;
+    const R den = r2 * r2 + a2 * z2;
+    const R rho = r2 + a2;
+    const R inv_den = rcpR<R>(den);
+    const R inv_rho = rcpR<R>(rho);
+    const R H = M * r3 * inv_den;
+    const R u = X * px + Y * py;
+    const R w = Y * px - X * py;
+    const R q1 = (r * u + a * w) * inv_rho;
+    const R q2 = Z * pz * inv_r;
+    const R s = q1 + q2 - pt;
+    const R ths = R(2) * H * s;
+    const R k0 = (r * X + a * Y) * inv_rho;
+    const R k1 = (r * Y - a * X) * inv_rho;
+    const R k2 = Z * inv_r;
+    two_hs = double(ths);
+    corr[0] = double(-(ths * k0));
+    corr[1] = double(-(ths * k1));
+    corr[2] = double(-(ths * k2));
+    const R drc = r3 * inv_den;
+    const R inv_den2 = inv_den * inv_den;
+    const R hb = M * r2 * (R(3) * a2 * z2 - r2 * r2) * inv_den2;
+    const R C = (u - R(2) * r * q1) * inv_rho - q2 * inv_r;
+    const R ss = s * s;
+    const R G = ss * hb + ths * C;
+    const R Gd = G * drc;
+    dp[0] = double(Gd * X + ths * ((r * px - a * py) * inv_rho));
+    dp[1] = double(Gd * Y + ths * ((r * py + a * px) * inv_rho));
+    dp[2] = double(G * (r * rho * Z * inv_den) + ths * (pz * inv_r) - ss * (R(2) * M * a2 * r3 * Z * inv_den2));
     Field f;
-    f.r = r;
-    f.H = H;
-    f.k0 = k0;
-    f.k1 = k1;
-    f.k2 = k2;
+    f.r = double(r);
+    f.H = double(H);
+    f.k0 = double(k0);
+    f.k1 = double(k1);
+    f.k2 = double(k2);
     return f;
 }
 
 // Single-hole RHS for y = (t, x, y, z, p_t, p_x, p_y, p_z); d[4] = 0.
+template <class R = double>
 __device__ __forceinline__ Field rhs_kerr(const double y[8], double M, double a, double a2, double d[8]) {
     double two_hs, corr[3], dpp[3];
-    const Field f = ks_terms(y[1], y[2], y[3], y[4], y[5], y[6], y[7], M, a, a2, two_hs, corr, dpp);
+    const Field f = ks_terms<R>(y[1], y[2], y[3], y[4], y[5], y[6], y[7], M, a, a2, two_hs, corr, dpp);
     d[0] = -y[4] + two_hs;
     d[1] = y[5] + corr[0];
     d[2] = y[6] + corr[1];

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -124,7 +125,7 @@ void fill_params(const rt_metric* mt, const rt_integrator* ig, const rt_terminat
     P.max_steps = ig->max_steps;
     P.precision = ig->precision;
     const double sw = std::max(ig->mixed_switch_radius, 3.0 * P.rh);
-    P.switch_r2 = sw * sw;
+    P.switch_r = sw;
 }
 
 double kerr_isco_host(double m, double a_spin) {
@@ -316,7 +317,7 @@ struct rt_context {
     rt::DeviceResults res;
     size_t n_last = 0;
     rt_stats stats{};
-    uint32_t block = 256;
+    rt::LaunchShape shape;
     std::mutex mu;
 };
 
@@ -356,7 +357,7 @@ rt_status run_trace(rt_context* ctx, const TraceParams& P, size_t count, size_t 
     s.rejected_steps = cnt[1];
     s.rhs_evals = cnt[2];
     s.fp32_steps = cnt[3];
-    s.block_size = ctx->block;
+    s.block_size = ctx->shape.block;
     s.grid_size = 0;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RT_OK;
@@ -433,6 +434,9 @@ rt_status rt_context_create(int device, rt_context** out) {
     rt_context* ctx = new (std::nothrow) rt_context();
     if (!ctx) return fail(RT_ERR_OOM, "context allocation");
     ctx->device = device;
+    // Launch-shape override for experiments: RTSIM_BLOCK (64..256), RTSIM_WAVES (1|2).
+    if (const char* b = std::getenv("RTSIM_BLOCK")) ctx->shape.block = static_cast<uint32_t>(std::atoi(b));
+    if (const char* w = std::getenv("RTSIM_WAVES")) ctx->shape.waves = static_cast<uint32_t>(std::atoi(w));
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
@@ -477,7 +481,7 @@ rt_status rt_trace_camera_resident(rt_context* ctx, const rt_camera* cam, const 
     fill_camera(cam, P);
     const size_t n = P.n_rays;
     st = run_trace(ctx, P, n, n, [&](const TraceParams& p, const OutPtrs& o) {
-        return launch_trace_camera(p, o, ctx->block, ctx->stream);
+        return launch_trace_camera(p, o, ctx->shape, ctx->stream);
     });
     if (st != RT_OK) return st;
     if (n_rays) *n_rays = n;
@@ -527,7 +531,7 @@ rt_status rt_trace_rays(rt_context* ctx, const double* x_soa, const double* p_so
     // with that leading dimension and copy back the first n of each row.
     const size_t ld = ctx->res.capacity;
     st = run_trace(ctx, P, n, ld, [&](const TraceParams& p, const OutPtrs& o) {
-        return launch_trace_rays(p, o, ctx->block, ctx->stream);
+        return launch_trace_rays(p, o, ctx->shape, ctx->stream);
     });
     if (st != RT_OK) return st;
     return copy_results_to_host(ctx->res.ptrs, n, out, ctx->stream);

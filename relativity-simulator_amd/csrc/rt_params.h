@@ -32,7 +32,7 @@ struct TraceParams {
     uint32_t pad0;
     // ---- integrator
     double rtol, atol, h0, h_min, h_max, inv_tol;
-    double switch_r2;   // mixed precision: FP32 RHS where r^2 > switch_r2
+    double switch_r;    // mixed precision: FP32 RHS for steps starting at r > switch_r
     uint32_t max_steps;
     uint32_t precision; // rt_precision
     // ---- camera (per-frame constants; identical for every pixel)
