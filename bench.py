@@ -80,6 +80,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=SIDE)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--path", choices=["batch", "plain"], default="batch",
+                    help="batch: the four frames in one persistent work-queue kernel (hipGraph replay); "
+                         "plain: one kernel per frame, one lane per pixel")
     ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -93,20 +96,32 @@ def main() -> int:
     ctx = rtsim.Context(0)
     print(f"bench: {len(reqs)} frames of {args.side}^2, precision {args.precision}, "
           f"warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
-    for w in range(args.warmup):
+    batch = [rtsim.make_request(*r) for r in reqs]
+
+    def one_step():
+        """Trace the four frames; returns [(rays, stats)] per kernel launch."""
+        if args.path == "batch":
+            ctx.trace_batch(batch, fetch=False)
+            st = ctx.stats()
+            return [(st.rays, st)]
+        out = []
         for r in reqs:
-            ctx.trace_camera_resident(*r)
+            n = ctx.trace_camera_resident(*r)
+            out.append((n, ctx.stats()))
+        return out
+
+    for w in range(args.warmup):
+        one_step()
         print(f"bench: warmup step {w + 1} done", file=sys.stderr, flush=True)
     kernel_ms = 0.0
     flops = 0.0
     rays = 0
     launches = 0
     acc = rej = rhs = 0
+    shape = None
     t0 = time.perf_counter()
     for s in range(args.steps):
-        for r in reqs:
-            n = ctx.trace_camera_resident(*r)
-            st = ctx.stats()
+        for n, st in one_step():
             kernel_ms += st.kernel_ms
             flops += st.rhs_evals * FLOP_RHS + (st.accepted_steps + st.rejected_steps) * FLOP_ATTEMPT
             rays += n
@@ -114,6 +129,7 @@ def main() -> int:
             acc += st.accepted_steps
             rej += st.rejected_steps
             rhs += st.rhs_evals
+            shape = (st.block_size, st.grid_size)
         print(f"bench: step {s + 1} done", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     value = rays / wall
@@ -135,7 +151,8 @@ def main() -> int:
         "config": {"workload": f"kerr_{args.side}sq_4spins_i60_robs1000_fov20", "side": args.side,
                    "spins": list(SPINS), "precision": args.precision, "rays_per_step": rays // args.steps,
                    "mean_accepted_steps": acc / rays, "mean_rejected_steps": rej / rays,
-                   "rhs_evals_per_ray": rhs / rays, "kernel_ms_per_frame": kernel_ms / launches},
+                   "rhs_evals_per_ray": rhs / rays, "path": args.path, "kernel_ms_per_launch": kernel_ms / launches,
+                   "launches_per_step": launches // args.steps, "block": shape[0], "grid": shape[1]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
     }

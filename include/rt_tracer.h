@@ -212,9 +212,29 @@ rt_status rt_trace_camera_resident(rt_context* ctx, const rt_camera* cam, const 
 /* Copy the device-resident results of the last resident trace to the host. */
 rt_status rt_fetch_results(rt_context* ctx, rt_results* out);
 
+/* ------------------------------------------------------- launch shapes */
+typedef struct rt_launch_shape {
+    uint32_t block;       /* threads per workgroup: 64, 128, 192 or 256          */
+    uint32_t waves;       /* occupancy the kernel is built for: 1 or 2 waves/SIMD */
+    uint32_t refill_min;  /* work queue: idle lanes that trigger a refill (1..64) */
+    uint32_t reserved;
+} rt_launch_shape;
+
+/* Shapes used by the per-frame kernels (plain) and the work-queue kernel. */
+rt_status rt_get_launch_shape(const rt_context* ctx, rt_launch_shape* plain, rt_launch_shape* queue);
+rt_status rt_set_launch_shape(rt_context* ctx, const rt_launch_shape* plain, const rt_launch_shape* queue);
+/* Time the work-queue kernel over candidate shapes on the given sample
+ * requests (single-Kerr requests only), keep the fastest in the context and
+ * return it in *best. */
+rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample, rt_launch_shape* best);
+
 /* ------------------------------------------------------- batched frames */
 /* Trace n_req camera requests of different sizes and tolerances in one
- * persistent kernel fed by a device work queue. outs[i] receives request i. */
+ * persistent kernel fed by a device work queue (superposition requests run
+ * as their own kernels in the same graph). The launch sequence is captured
+ * into a hipGraph that is replayed while the batch layout stays the same.
+ * outs[i] receives request i; outs may be NULL to keep the results resident
+ * (then only the statistics are updated). */
 rt_status rt_trace_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, rt_results* outs);
 
 /* ------------------------------------------------------------- serving */

@@ -948,3 +948,46 @@ extern "C" void rto_flop_counts(double out[4]) {
     out[2] = static_cast<double>(rhs_ops);          // six RHS evaluations inside an attempt
     out[3] = 0.0;
 }
+
+// Integrate y0 over an affine length L (sign of L = direction) with the same
+// DP45 steps, projection and controller as trace_ray but no events; the last
+// step is shortened to land on L exactly. Single hole only. Returns accepted
+// steps (negative if the step size fell below h_min).
+extern "C" int rto_integrate_to(const double y0[8], double M, double a, double L, double rtol, double atol,
+                                double h0, double y_out[8]) {
+    rt_metric mt{};
+    mt.mass = M;
+    mt.spin = a;
+    rt_termination tm{};
+    tm.horizon_eps = 1e-6;
+    tm.escape_radius = 1e30;
+    const rto::Spacetime st = rto::make_spacetime(&mt, &tm);
+    double y[8], y5[8], k[7][8];
+    std::memcpy(y, y0, sizeof(y));
+    auto f = [&](const double* yy, double* dd) { rto::rhs(st, yy, dd); };
+    f(y, k[0]);
+    const double sgn = (L < 0.0) ? -1.0 : 1.0;
+    const double Labs = std::fabs(L);
+    double lam = 0.0, ah = h0;
+    int acc = 0;
+    while (lam < Labs) {
+        if (ah < 1e-14) return -acc;
+        const bool last = (lam + ah >= Labs);
+        const double step = last ? (Labs - lam) : ah;
+        const double err = rto::dp45_attempt<double, 8>(f, y, sgn * step, rtol, atol, k, y5);
+        if (err <= 1.0) {
+            std::memcpy(y, y5, sizeof(y));
+            ++acc;
+            lam = last ? Labs : lam + step;
+            if (rto::project(st, y, 1e-12))
+                f(y, k[0]);
+            else
+                std::memcpy(k[0], k[6], sizeof(k[0]));
+            if (!last) ah = rto::dp45_next_step(step, err, true, 1e-14, 1e13);
+        } else {
+            ah = rto::dp45_next_step(step, err, false, 1e-14, 1e13);
+        }
+    }
+    std::memcpy(y_out, y, sizeof(y));
+    return acc;
+}

@@ -1,25 +1,735 @@
-// rt_batch.cpp — batched frames and the serving loop (rt_trace_batch, rt_server_*).
+// rt_batch.cpp — batched frames (rt_trace_batch), launch shapes and the
+// autotuner (rt_*_launch_shape, rt_autotune), and the serving loop (rt_server_*).
+//
+// A batch is laid out as one work queue: every single-Kerr request becomes a
+// contiguous range of queue indices (8x8-tile order for cameras) served by the
+// persistent k_trace_queue kernel; superposition requests run as their own
+// k_trace_multi kernels after it. The launch sequence of a batch is captured
+// into a hipGraph and replayed while the batch layout (pointers, sizes,
+// by-value kernel arguments, launch shape) stays the same.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/rt_tracer.h"
 #include "rt_host.h"
+#include "rt_kernels.h"
+
+namespace rt {
+
+namespace {
+
+constexpr size_t kCounterBytes = 256;  // 16 statistics counters + queue head at [8]
+constexpr uint32_t kQueueHead = 8;
+
+size_t al256(size_t v) { return (v + 255) & ~size_t(255); }
+
+struct BatchPlan {
+    std::vector<uint32_t> q_req;         // request index of each queue entry
+    std::vector<TraceParams> q_params;
+    std::vector<uint64_t> q_offset;      // n_q + 1
+    std::vector<uint32_t> m_req;         // request index of each superposition entry
+    std::vector<TraceParams> m_params;
+    std::vector<MultiDev> m_tables;
+    std::vector<size_t> n_rays;          // per request
+    bool mixed = false;
+    size_t result_bytes = 0, table_bytes = 0;
+};
+
+// Validate and lay out a batch (no device work).
+rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan) {
+    plan = BatchPlan{};
+    plan.n_rays.resize(n);
+    plan.q_offset.push_back(0);
+    for (size_t i = 0; i < n; ++i) {
+        const rt_request& r = reqs[i];
+        std::string why;
+        if (!valid_request(&r.metric, &r.integrator, &r.termination, why) || !valid_camera(&r.camera, why))
+            return fail(RT_ERR_INVALID_ARG, "request " + std::to_string(i) + ": " + why);
+        TraceParams P;
+        fill_params(&r.metric, &r.integrator, &r.termination, P);
+        if (P.n_src >= 2) {
+            MultiDev T;
+            fill_sources(&r.metric, &r.termination, T);
+            fill_camera(&r.camera, P, &T);
+            plan.m_req.push_back(static_cast<uint32_t>(i));
+            plan.m_params.push_back(P);
+            plan.m_tables.push_back(T);
+        } else {
+            fill_camera(&r.camera, P, nullptr);
+            plan.q_req.push_back(static_cast<uint32_t>(i));
+            plan.q_params.push_back(P);
+            const uint64_t tiles_y = (P.n_rays / P.width + 7) / 8;
+            plan.q_offset.push_back(plan.q_offset.back() + static_cast<uint64_t>(P.tiles_x) * tiles_y * 64);
+            plan.mixed = plan.mixed || P.mixed;
+        }
+        plan.n_rays[i] = P.n_rays;
+        plan.result_bytes += results_bytes(P.n_rays, false);
+    }
+    const size_t nq = plan.q_params.size(), nm = plan.m_params.size();
+    plan.table_bytes = kCounterBytes + al256(nq * sizeof(TraceParams)) + al256(nq * sizeof(OutPtrs)) +
+                       al256((nq + 1) * sizeof(uint64_t)) + al256(nm * sizeof(MultiDev));
+    return RT_OK;
+}
+
+// Device pointers of a placed batch.
+struct Placed {
+    std::vector<OutPtrs> outs;           // per request (index = request index)
+    QueueDev Q{};
+    unsigned long long* counters = nullptr;
+    TraceParams* d_params = nullptr;
+    OutPtrs* d_outs = nullptr;
+    uint64_t* d_offset = nullptr;
+    MultiDev* d_tables = nullptr;
+};
+
+void place_batch(const BatchPlan& plan, char* results, char* tables, const LaunchShape& shape, Placed& pl) {
+    const size_t n = plan.n_rays.size();
+    pl.outs.assign(n, OutPtrs{});
+    char* cur = results;
+    char* tb = tables;
+    pl.counters = reinterpret_cast<unsigned long long*>(tb);
+    tb += kCounterBytes;
+    for (size_t i = 0; i < n; ++i) {
+        pl.outs[i] = carve_results(cur, plan.n_rays[i], false);
+        pl.outs[i].counters = pl.counters;
+    }
+    const size_t nq = plan.q_params.size(), nm = plan.m_params.size();
+    pl.d_params = reinterpret_cast<TraceParams*>(tb);
+    tb += al256(nq * sizeof(TraceParams));
+    pl.d_outs = reinterpret_cast<OutPtrs*>(tb);
+    tb += al256(nq * sizeof(OutPtrs));
+    pl.d_offset = reinterpret_cast<uint64_t*>(tb);
+    tb += al256((nq + 1) * sizeof(uint64_t));
+    pl.d_tables = reinterpret_cast<MultiDev*>(tb);
+    pl.Q.reqs = pl.d_params;
+    pl.Q.outs = pl.d_outs;
+    pl.Q.req_offset = pl.d_offset;
+    pl.Q.next = pl.counters + kQueueHead;
+    pl.Q.counters = pl.counters;
+    pl.Q.total = plan.q_offset.back();
+    pl.Q.n_req = static_cast<uint32_t>(nq);
+    pl.Q.refill_min = std::max<uint32_t>(1, std::min<uint32_t>(shape.refill_min, 64));
+    (void)nm;
+}
+
+// Host->device copies of the batch tables (outside any captured graph).
+rt_status upload_batch(const BatchPlan& plan, const Placed& pl, hipStream_t s) {
+    const size_t nq = plan.q_params.size(), nm = plan.m_params.size();
+    if (nq) {
+        std::vector<OutPtrs> qo(nq);
+        for (size_t j = 0; j < nq; ++j) qo[j] = pl.outs[plan.q_req[j]];
+        RT_HIP(hipMemcpyAsync(pl.d_params, plan.q_params.data(), nq * sizeof(TraceParams), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(pl.d_outs, qo.data(), nq * sizeof(OutPtrs), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(pl.d_offset, plan.q_offset.data(), (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              s));
+        // the kernels read these tables; wait for the pageable copies to land
+        RT_HIP(hipStreamSynchronize(s));
+    }
+    if (nm) {
+        RT_HIP(hipMemcpyAsync(pl.d_tables, plan.m_tables.data(), nm * sizeof(MultiDev), hipMemcpyHostToDevice, s));
+        RT_HIP(hipStreamSynchronize(s));
+    }
+    return RT_OK;
+}
+
+// The launch sequence of a batch (captured into a graph by rt_trace_batch).
+hipError_t launch_batch(const BatchPlan& plan, const Placed& pl, const LaunchShape& shape, uint32_t grid,
+                        hipStream_t s) {
+    hipError_t e = launch_zero_u64(pl.counters, kCounterBytes / 8, s);
+    if (e != hipSuccess) return e;
+    if (!plan.q_params.empty()) {
+        e = launch_trace_queue(pl.Q, shape, plan.mixed, grid, s);
+        if (e != hipSuccess) return e;
+    }
+    for (size_t j = 0; j < plan.m_params.size(); ++j) {
+        e = launch_trace_multi(plan.m_params[j], pl.outs[plan.m_req[j]], pl.d_tables + j, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Signature of everything a captured graph bakes in.
+std::vector<unsigned char> graph_key(const BatchPlan& plan, const Placed& pl, const LaunchShape& shape) {
+    std::vector<unsigned char> k;
+    auto put = [&](const void* p, size_t b) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), c, c + b);
+    };
+    put(&pl.Q, sizeof(pl.Q));
+    put(&shape, sizeof(shape));
+    const bool mixed = plan.mixed;
+    put(&mixed, sizeof(mixed));
+    for (size_t j = 0; j < plan.m_params.size(); ++j) {
+        put(&plan.m_params[j], sizeof(TraceParams));
+        put(&pl.outs[plan.m_req[j]], sizeof(OutPtrs));
+    }
+    return k;
+}
+
+}  // namespace
+
+// Graph cache of a context (one batch layout).
+struct GraphCache {
+    std::vector<unsigned char> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key.clear();
+    }
+    ~GraphCache() { reset(); }
+};
+
+static std::mutex g_graph_mu;
+static std::map<const rt_context*, GraphCache*> g_graphs;
+
+void forget_graphs(const rt_context* ctx) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find(ctx);
+    if (it != g_graphs.end()) {
+        delete it->second;
+        g_graphs.erase(it);
+    }
+}
+
+static GraphCache& graph_cache(const rt_context* ctx) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    GraphCache*& g = g_graphs[ctx];
+    if (!g) g = new GraphCache();
+    return *g;
+}
+
+// Run a batch on the context: place, upload, replay or capture the graph, time it.
+static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, rt_results* outs) {
+    const auto t0 = std::chrono::steady_clock::now();
+    RT_HIP(hipSetDevice(ctx->device));
+    BatchPlan plan;
+    rt_status st = plan_batch(reqs, n_req, plan);
+    if (st != RT_OK) return st;
+    st = ctx->arena.ensure(std::max<size_t>(plan.result_bytes, 256));
+    if (st != RT_OK) return st;
+    st = ctx->aux.ensure(plan.table_bytes + sizeof(MultiDev) + 256);
+    if (st != RT_OK) return st;
+    Placed pl;
+    place_batch(plan, static_cast<char*>(ctx->arena.base), static_cast<char*>(ctx->aux.base), ctx->qshape, pl);
+    st = upload_batch(plan, pl, ctx->stream);
+    if (st != RT_OK) return st;
+    GraphCache& gc = graph_cache(ctx);
+    (void)0;
+    std::vector<unsigned char> key = graph_key(plan, pl, ctx->qshape);
+    if (!gc.exec || gc.key != key) {
+        gc.reset();
+        uint32_t grid = 0;
+        RT_HIP(queue_grid(ctx->qshape, plan.mixed, &grid));
+        RT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = launch_batch(plan, pl, ctx->qshape, grid, ctx->stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (le != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(le, "batch launch (capture)");
+        }
+        if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+        gc.graph = g;
+        RT_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+        gc.key = key;
+    }
+    RT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    RT_HIP(hipGraphLaunch(gc.exec, ctx->stream));
+    RT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    RT_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    RT_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+    RT_HIP(hipMemcpy(cnt, pl.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    size_t total = 0;
+    for (size_t i = 0; i < n_req; ++i) total += plan.n_rays[i];
+    if (outs)
+        for (size_t i = 0; i < n_req; ++i) {
+            st = copy_results_to_host(pl.outs[i], plan.n_rays[i], &outs[i], ctx->stream);
+            if (st != RT_OK) return st;
+        }
+    ctx->last = n_req ? pl.outs[0] : OutPtrs{};
+    ctx->n_last = n_req ? plan.n_rays[0] : 0;
+    rt_stats& s = ctx->stats;
+    s.kernel_ms = ms;
+    s.rays = total;
+    s.accepted_steps = cnt[0];
+    s.rejected_steps = cnt[1];
+    s.rhs_evals = cnt[2];
+    s.fp32_steps = cnt[3];
+    s.block_size = ctx->qshape.block;
+    uint32_t grid = 0;
+    (void)queue_grid(ctx->qshape, plan.mixed, &grid);
+    s.grid_size = grid;
+    s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RT_OK;
+}
+
+}  // namespace rt
 
 using namespace rt;
 
+// ===================================================================== server
+namespace {
+
+enum class ReqState { Pending, Running, Done, Cancelled, Failed };
+
+struct ServerReq {
+    rt_request req;
+    size_t n = 0, bytes = 0;
+    ReqState state = ReqState::Pending;
+    rt_status err = RT_OK;
+    bool cancel = false;
+    std::vector<uint32_t> cls, steps, rejected;
+    std::vector<double> x, p, bl, lambda, g;
+};
+
+struct Inflight {
+    std::vector<uint64_t> ids;
+    size_t off = 0, bytes = 0;
+    int stream = 0;
+    hipEvent_t done = nullptr;
+    Placed pl;
+    std::vector<size_t> n_rays;
+};
+
+// First-fit allocator over [0, size) in 256-byte units.
+struct PoolAlloc {
+    std::map<size_t, size_t> free_;  // offset -> bytes
+    void init(size_t size) {
+        free_.clear();
+        free_[0] = size;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = al256(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t noff = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[noff] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = al256(bytes);
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+}  // namespace
+
+struct rt_server {
+    int device = 0;
+    rt_server_config cfg{};
+    void* pool = nullptr;
+    size_t pool_bytes = 0;
+    PoolAlloc alloc;
+    std::vector<hipStream_t> streams;
+    std::vector<bool> stream_busy;
+    LaunchShape qshape;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_id = 1;
+    std::map<uint64_t, ServerReq> reqs;
+    std::deque<uint64_t> pending;
+    std::vector<Inflight*> inflight;
+    size_t running_reqs = 0;
+
+    static size_t footprint(size_t n_rays, bool multi) {
+        return results_bytes(n_rays, false) + kCounterBytes + 3 * 256 + sizeof(TraceParams) + sizeof(OutPtrs) +
+               (multi ? sizeof(MultiDev) : 0) + 4096;
+    }
+
+    void run();
+    void launch_locked(std::unique_lock<std::mutex>& lk);
+    void complete(Inflight* b, std::unique_lock<std::mutex>& lk);
+};
+
+void rt_server::launch_locked(std::unique_lock<std::mutex>& lk) {
+    // pick a free stream
+    int sidx = -1;
+    for (size_t i = 0; i < streams.size(); ++i)
+        if (!stream_busy[i]) {
+            sidx = static_cast<int>(i);
+            break;
+        }
+    if (sidx < 0 || pending.empty()) return;
+    // admit pending requests FIFO while they fit in one pool block
+    std::vector<uint64_t> take;
+    std::vector<rt_request> batch;
+    size_t bytes = 0;
+    while (!pending.empty() && running_reqs + take.size() < cfg.max_inflight) {
+        const uint64_t id = pending.front();
+        ServerReq& r = reqs[id];
+        if (r.cancel) {
+            pending.pop_front();
+            r.state = ReqState::Cancelled;
+            cv_done.notify_all();
+            continue;
+        }
+        size_t off_probe = 0;
+        const size_t want = bytes + r.bytes;
+        if (!alloc.alloc(want, &off_probe)) break;
+        alloc.release(off_probe, want);
+        take.push_back(id);
+        batch.push_back(r.req);
+        bytes = want;
+        pending.pop_front();
+    }
+    if (take.empty()) return;
+    Inflight* b = new Inflight();
+    b->ids = take;
+    b->stream = sidx;
+    b->bytes = bytes;
+    alloc.alloc(bytes, &b->off);
+    for (uint64_t id : take) reqs[id].state = ReqState::Running;
+    running_reqs += take.size();
+    stream_busy[sidx] = true;
+    lk.unlock();
+    // plan + place + upload + launch outside the lock
+    BatchPlan plan;
+    rt_status st = plan_batch(batch.data(), batch.size(), plan);
+    hipError_t he = hipSuccess;
+    if (st == RT_OK) {
+        char* base = static_cast<char*>(pool) + b->off;
+        place_batch(plan, base, base + al256(plan.result_bytes), qshape, b->pl);
+        b->n_rays = plan.n_rays;
+        hipStream_t s = streams[sidx];
+        st = upload_batch(plan, b->pl, s);
+        if (st == RT_OK) he = launch_batch(plan, b->pl, qshape, 0, s);
+        if (st == RT_OK && he == hipSuccess) he = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+        if (st == RT_OK && he == hipSuccess) he = hipEventRecord(b->done, s);
+    }
+    lk.lock();
+    if (st != RT_OK || he != hipSuccess) {
+        for (uint64_t id : take) {
+            ServerReq& r = reqs[id];
+            r.state = ReqState::Failed;
+            r.err = (st != RT_OK) ? st : RT_ERR_DEVICE;
+        }
+        running_reqs -= take.size();
+        stream_busy[sidx] = false;
+        alloc.release(b->off, b->bytes);
+        if (b->done) (void)hipEventDestroy(b->done);
+        delete b;
+        cv_done.notify_all();
+        return;
+    }
+    inflight.push_back(b);
+}
+
+void rt_server::complete(Inflight* b, std::unique_lock<std::mutex>& lk) {
+    // copy results out (outside the lock), then release the pool block
+    std::vector<ServerReq*> rs;
+    for (uint64_t id : b->ids) rs.push_back(&reqs[id]);
+    lk.unlock();
+    hipStream_t s = streams[b->stream];
+    rt_status st = RT_OK;
+    for (size_t i = 0; i < rs.size(); ++i) {
+        ServerReq& r = *rs[i];
+        const size_t n = b->n_rays[i];
+        r.cls.resize(n);
+        r.steps.resize(n);
+        r.rejected.resize(n);
+        r.x.resize(4 * n);
+        r.p.resize(4 * n);
+        r.bl.resize(3 * n);
+        r.lambda.resize(n);
+        r.g.resize(n);
+        rt_results o{r.cls.data(), r.x.data(), r.p.data(), r.bl.data(), r.lambda.data(),
+                     r.g.data(),   r.steps.data(), r.rejected.data()};
+        if (st == RT_OK) st = copy_results_to_host(b->pl.outs[i], n, &o, s);
+    }
+    lk.lock();
+    for (ServerReq* r : rs) {
+        if (r->cancel)
+            r->state = ReqState::Cancelled;
+        else if (st != RT_OK) {
+            r->state = ReqState::Failed;
+            r->err = st;
+        } else
+            r->state = ReqState::Done;
+    }
+    running_reqs -= b->ids.size();
+    stream_busy[b->stream] = false;
+    alloc.release(b->off, b->bytes);
+    (void)hipEventDestroy(b->done);
+    delete b;
+    cv_done.notify_all();
+}
+
+void rt_server::run() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (!stop) {
+        bool progressed = false;
+        for (size_t i = 0; i < inflight.size();) {
+            Inflight* b = inflight[i];
+            if (hipEventQuery(b->done) == hipSuccess) {
+                inflight.erase(inflight.begin() + static_cast<long>(i));
+                complete(b, lk);
+                progressed = true;
+            } else {
+                ++i;
+            }
+        }
+        const size_t before = inflight.size();
+        launch_locked(lk);
+        if (inflight.size() != before) progressed = true;
+        if (!progressed) cv_work.wait_for(lk, std::chrono::microseconds(200));
+    }
+    // drain: wait for every batch still on the device
+    while (!inflight.empty()) {
+        Inflight* b = inflight.back();
+        inflight.pop_back();
+        lk.unlock();
+        (void)hipEventSynchronize(b->done);
+        lk.lock();
+        complete(b, lk);
+    }
+}
+
 extern "C" {
 
-rt_status rt_trace_batch(rt_context*, const rt_request*, size_t, rt_results*) {
-    return fail(RT_ERR_INVALID_ARG, "rt_trace_batch: not available in this build");
+rt_status rt_get_launch_shape(const rt_context* ctx, rt_launch_shape* plain, rt_launch_shape* queue) {
+    if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
+    if (plain) *plain = rt_launch_shape{ctx->shape.block, ctx->shape.waves, ctx->shape.refill_min, 0};
+    if (queue) *queue = rt_launch_shape{ctx->qshape.block, ctx->qshape.waves, ctx->qshape.refill_min, 0};
+    return RT_OK;
 }
-rt_status rt_server_create(int, const rt_server_config*, rt_server** out) {
-    if (out) *out = nullptr;
-    return fail(RT_ERR_INVALID_ARG, "rt_server: not available in this build");
+
+static bool shape_valid(const rt_launch_shape* s) {
+    return s->block >= 64 && s->block <= 256 && (s->block % 64) == 0 && (s->waves == 1 || s->waves == 2) &&
+           s->refill_min >= 1 && s->refill_min <= 64;
 }
-rt_status rt_server_destroy(rt_server*) { return RT_OK; }
-rt_status rt_server_submit(rt_server*, const rt_request*, uint64_t*) { return fail(RT_ERR_INVALID_ARG, "no server"); }
-rt_status rt_server_poll(rt_server*, uint64_t, int*) { return fail(RT_ERR_INVALID_ARG, "no server"); }
-rt_status rt_server_wait(rt_server*, uint64_t, int64_t) { return fail(RT_ERR_INVALID_ARG, "no server"); }
-rt_status rt_server_take(rt_server*, uint64_t, rt_results*) { return fail(RT_ERR_INVALID_ARG, "no server"); }
-rt_status rt_server_cancel(rt_server*, uint64_t) { return fail(RT_ERR_INVALID_ARG, "no server"); }
+
+rt_status rt_set_launch_shape(rt_context* ctx, const rt_launch_shape* plain, const rt_launch_shape* queue) {
+    if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
+    if ((plain && !shape_valid(plain)) || (queue && !shape_valid(queue)))
+        return fail(RT_ERR_INVALID_ARG, "launch shape: block in {64,128,192,256}, waves in {1,2}, refill_min 1..64");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (plain) ctx->shape = LaunchShape{plain->block, plain->waves, plain->refill_min};
+    if (queue) ctx->qshape = LaunchShape{queue->block, queue->waves, queue->refill_min};
+    return RT_OK;
+}
+
+rt_status rt_trace_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, rt_results* outs) {
+    if (!ctx || !reqs || n_req == 0) return fail(RT_ERR_INVALID_ARG, "null context/requests or n_req == 0");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return run_batch(ctx, reqs, n_req, outs);
+}
+
+rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample, rt_launch_shape* best) {
+    if (!ctx || !sample || n_sample == 0) return fail(RT_ERR_INVALID_ARG, "null context/sample");
+    for (size_t i = 0; i < n_sample; ++i)
+        if (sample[i].metric.n_sources >= 2) return fail(RT_ERR_INVALID_ARG, "autotune takes single-Kerr requests");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const uint32_t blocks[] = {64, 128, 256};
+    const uint32_t waves[] = {1, 2};
+    const uint32_t refills[] = {8, 16, 32};
+    LaunchShape keep = ctx->qshape, win = ctx->qshape;
+    double best_ms = 1e300;
+    for (uint32_t b : blocks)
+        for (uint32_t w : waves)
+            for (uint32_t r : refills) {
+                ctx->qshape = LaunchShape{b, w, r};
+                rt_status st = run_batch(ctx, sample, n_sample, nullptr);  // capture + run
+                if (st == RT_OK) st = run_batch(ctx, sample, n_sample, nullptr);  // replay, timed
+                if (st != RT_OK) {
+                    ctx->qshape = keep;
+                    return st;
+                }
+                if (ctx->stats.kernel_ms < best_ms) {
+                    best_ms = ctx->stats.kernel_ms;
+                    win = ctx->qshape;
+                }
+            }
+    ctx->qshape = win;
+    ctx->tuned = true;
+    if (best) *best = rt_launch_shape{win.block, win.waves, win.refill_min, 0};
+    return RT_OK;
+}
+
+rt_status rt_server_create(int device, const rt_server_config* cfg, rt_server** out) {
+    if (!out || !cfg) return fail(RT_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->hbm_budget_bytes < (1u << 20) || cfg->max_inflight == 0)
+        return fail(RT_ERR_INVALID_ARG, "server: hbm_budget_bytes >= 1 MiB and max_inflight > 0");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(RT_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(RT_ERR_INVALID_ARG, "device index out of range");
+    rt_server* s = new (std::nothrow) rt_server();
+    if (!s) return fail(RT_ERR_OOM, "server allocation");
+    s->device = device;
+    s->cfg = *cfg;
+    RT_HIP(hipSetDevice(device));
+    s->pool_bytes = al256(cfg->hbm_budget_bytes);
+    hipError_t e = hipMalloc(&s->pool, s->pool_bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(e, "hipMalloc(server pool)");
+    }
+    s->alloc.init(s->pool_bytes);
+    const int n_streams = 4;  // one per hardware queue
+    for (int i = 0; i < n_streams; ++i) {
+        hipStream_t st = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) break;
+        s->streams.push_back(st);
+        s->stream_busy.push_back(false);
+    }
+    if (e != hipSuccess) {
+        rt_server_destroy(s);
+        return hip_fail(e, "server streams");
+    }
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return RT_OK;
+}
+
+rt_status rt_server_destroy(rt_server* s) {
+    if (!s) return RT_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->device);
+    for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
+    if (s->pool) (void)hipFree(s->pool);
+    delete s;
+    return RT_OK;
+}
+
+rt_status rt_server_submit(rt_server* s, const rt_request* req, uint64_t* id) {
+    if (!s || !req || !id) return fail(RT_ERR_INVALID_ARG, "null argument");
+    std::string why;
+    if (!valid_request(&req->metric, &req->integrator, &req->termination, why) || !valid_camera(&req->camera, why))
+        return fail(RT_ERR_INVALID_ARG, why);
+    TraceParams P;
+    fill_params(&req->metric, &req->integrator, &req->termination, P);
+    fill_camera(&req->camera, P, nullptr);
+    const size_t bytes = rt_server::footprint(P.n_rays, req->metric.n_sources >= 2);
+    if (bytes > s->pool_bytes)
+        return fail(RT_ERR_BUSY, "request footprint " + std::to_string(bytes) + " B exceeds the HBM budget");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const uint64_t rid = s->next_id++;
+    ServerReq& r = s->reqs[rid];
+    r.req = *req;
+    r.n = P.n_rays;
+    r.bytes = bytes;
+    s->pending.push_back(rid);
+    *id = rid;
+    s->cv_work.notify_all();
+    return RT_OK;
+}
+
+rt_status rt_server_poll(rt_server* s, uint64_t id, int* done) {
+    if (!s || !done) return fail(RT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
+    const ReqState st = it->second.state;
+    *done = (st == ReqState::Done || st == ReqState::Cancelled || st == ReqState::Failed) ? 1 : 0;
+    return RT_OK;
+}
+
+rt_status rt_server_wait(rt_server* s, uint64_t id, int64_t timeout_ms) {
+    if (!s) return fail(RT_ERR_INVALID_ARG, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto finished = [&]() {
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) return true;
+        const ReqState st = it->second.state;
+        return st == ReqState::Done || st == ReqState::Cancelled || st == ReqState::Failed;
+    };
+    if (timeout_ms < 0)
+        s->cv_done.wait(lk, finished);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), finished))
+        return fail(RT_ERR_TIMEOUT, "request not finished before the deadline");
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
+    if (it->second.state == ReqState::Cancelled) return fail(RT_ERR_CANCELLED, "request cancelled");
+    if (it->second.state == ReqState::Failed) return fail(it->second.err, "request failed");
+    return RT_OK;
+}
+
+rt_status rt_server_take(rt_server* s, uint64_t id, rt_results* out) {
+    if (!s || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
+    ServerReq& r = it->second;
+    if (r.state == ReqState::Cancelled) {
+        s->reqs.erase(it);
+        return fail(RT_ERR_CANCELLED, "request cancelled");
+    }
+    if (r.state == ReqState::Failed) {
+        const rt_status e = r.err;
+        s->reqs.erase(it);
+        return fail(e, "request failed");
+    }
+    if (r.state != ReqState::Done) return fail(RT_ERR_BUSY, "request not finished");
+    const size_t n = r.n;
+    if (out->cls) std::memcpy(out->cls, r.cls.data(), n * 4);
+    if (out->x) std::memcpy(out->x, r.x.data(), n * 32);
+    if (out->p) std::memcpy(out->p, r.p.data(), n * 32);
+    if (out->bl) std::memcpy(out->bl, r.bl.data(), n * 24);
+    if (out->lambda) std::memcpy(out->lambda, r.lambda.data(), n * 8);
+    if (out->g) std::memcpy(out->g, r.g.data(), n * 8);
+    if (out->steps) std::memcpy(out->steps, r.steps.data(), n * 4);
+    if (out->rejected) std::memcpy(out->rejected, r.rejected.data(), n * 4);
+    s->reqs.erase(it);
+    return RT_OK;
+}
+
+rt_status rt_server_cancel(rt_server* s, uint64_t id) {
+    if (!s) return fail(RT_ERR_INVALID_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
+    it->second.cancel = true;
+    if (it->second.state == ReqState::Pending) {
+        s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), id), s->pending.end());
+        it->second.state = ReqState::Cancelled;
+        s->cv_done.notify_all();
+    }
+    s->cv_work.notify_all();
+    return RT_OK;
+}
 
 }  // extern "C"

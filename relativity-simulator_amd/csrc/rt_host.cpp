@@ -1,12 +1,11 @@
 // rt_host.cpp — C++ host runtime of librtsim.so: parameter set-up from the
-// C-ABI structs, per-context device buffers and stream, timing, and the
-// extern "C" entry points declared in include/rt_tracer.h.
+// C-ABI structs, device buffers, timing, and the single-trace extern "C"
+// entry points of include/rt_tracer.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -33,14 +32,12 @@ rt_status hip_fail(hipError_t e, const char* what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define RT_HIP(call)                                         \
-    do {                                                     \
-        hipError_t e_ = (call);                              \
-        if (e_ != hipSuccess) return hip_fail(e_, #call);    \
-    } while (0)
+const char* last_error_text() { return g_last_error.c_str(); }
 
 // ------------------------------------------------------ host geometry
 namespace {
+
+constexpr double kPiH = 3.14159265358979323846;
 
 double horizon_radius(double m, double a) { return m + std::sqrt(std::max(m * m - a * a, 0.0)); }
 
@@ -66,7 +63,24 @@ HField ks_field_host(const double x[3], double M, double a) {
     return f;
 }
 
+void to_local_host(const SourceDev& s, const double x[3], double l[3]) {
+    const double d[3] = {x[0] - s.c[0], x[1] - s.c[1], x[2] - s.c[2]};
+    l[0] = d[0] * s.fx[0] + d[1] * s.fx[1] + d[2] * s.fx[2];
+    l[1] = d[0] * s.fy[0] + d[1] * s.fy[1] + d[2] * s.fy[2];
+    l[2] = d[0] * s.ax[0] + d[1] * s.ax[1] + d[2] * s.ax[2];
+}
+
 }  // namespace
+
+double kerr_isco_host(double m, double a_spin) {
+    const double a_star = std::clamp(a_spin / std::max(m, 1e-12), -0.9999999, 0.9999999);
+    const double abs_a = std::abs(a_star);
+    const double orbit_sign = (a_star >= 0.0) ? -1.0 : 1.0;
+    const double cbrt_term = std::cbrt(std::max(1.0 - abs_a * abs_a, 0.0));
+    const double z1 = 1.0 + cbrt_term * (std::cbrt(1.0 + abs_a) + std::cbrt(1.0 - abs_a));
+    const double z2 = std::sqrt(3.0 * abs_a * abs_a + z1 * z1);
+    return (3.0 + z2 + orbit_sign * std::sqrt(std::max((3.0 - z1) * (3.0 + z1 + 2.0 * z2), 0.0))) * m;
+}
 
 bool valid_request(const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm, std::string& why) {
     if (!mt || !ig || !tm) {
@@ -86,56 +100,22 @@ bool valid_request(const rt_metric* mt, const rt_integrator* ig, const rt_termin
         why = "termination: need escape_radius > 0 and horizon_eps >= 0";
         return false;
     }
-    if (!(mt->mass > 0.0) && mt->n_sources < 2) {
-        why = "metric: mass must be > 0";
-        return false;
-    }
     if (mt->n_sources > RT_MAX_SOURCES) {
         why = "metric: at most 16 sources";
         return false;
     }
+    if (mt->n_sources >= 2) {
+        for (uint32_t i = 0; i < mt->n_sources; ++i)
+            if (!(mt->src[i].mass > 0.0) || !std::isfinite(mt->src[i].position[0]) ||
+                !std::isfinite(mt->src[i].position[1]) || !std::isfinite(mt->src[i].position[2])) {
+                why = "metric: every source needs mass > 0 and a finite position";
+                return false;
+            }
+    } else if (!(mt->mass > 0.0)) {
+        why = "metric: mass must be > 0";
+        return false;
+    }
     return true;
-}
-
-// Metric, integrator and termination parts of the parameter block
-// (restating dispatch_fp64_scalar's set-up, render/software_compute_engine.hpp:3801-3809).
-void fill_params(const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm, TraceParams& P) {
-    std::memset(&P, 0, sizeof(P));
-    P.M = std::max(mt->mass, 1e-4);
-    P.a = std::clamp(mt->spin, -0.999 * P.M, 0.999 * P.M);
-    P.a2 = P.a * P.a;
-    P.rh = horizon_radius(P.M, P.a);
-    P.r_hz = P.rh + tm->horizon_eps;
-    P.r_esc = tm->escape_radius;
-    P.disk_spin = std::clamp(P.a, -0.999 * P.M, 0.999 * P.M);
-    P.disk_on = tm->disk_enabled ? 1u : 0u;
-    P.clamp_g = (tm->flags & RT_TERM_NO_G_CLAMP) ? 0u : 1u;
-    const double isco = kerr_isco_host(P.M, P.a);
-    P.disk_in = (tm->disk_inner > 0.0) ? tm->disk_inner : isco;
-    P.disk_out = std::max((tm->disk_outer > 0.0) ? tm->disk_outer : 20.0 * P.M, P.disk_in * 1.05);
-    P.rh102 = P.rh * 1.02;
-    P.near_r = 3.0 * (2.0 * P.M);
-    P.n_src = (mt->n_sources >= 2) ? mt->n_sources : 0u;
-    P.rtol = ig->rtol;
-    P.atol = ig->atol;
-    P.h0 = ig->h0;
-    P.h_min = ig->h_min;
-    P.h_max = ig->h_max;
-    P.inv_tol = ig->invariant_tol;
-    P.max_steps = ig->max_steps;
-    P.precision = ig->precision;
-    const double sw = std::max(ig->mixed_switch_radius, 3.0 * P.rh);
-    P.switch_r = sw;
-}
-
-double kerr_isco_host(double m, double a_spin) {
-    const double a_star = std::clamp(a_spin / std::max(m, 1e-12), -0.9999999, 0.9999999);
-    const double abs_a = std::abs(a_star);
-    const double orbit_sign = (a_star >= 0.0) ? -1.0 : 1.0;
-    const double cbrt_term = std::cbrt(std::max(1.0 - abs_a * abs_a, 0.0));
-    const double z1 = 1.0 + cbrt_term * (std::cbrt(1.0 + abs_a) + std::cbrt(1.0 - abs_a));
-    const double z2 = std::sqrt(3.0 * abs_a * abs_a + z1 * z1);
-    return (3.0 + z2 + orbit_sign * std::sqrt(std::max((3.0 - z1) * (3.0 + z1 + 2.0 * z2), 0.0))) * m;
 }
 
 bool valid_camera(const rt_camera* c, std::string& why) {
@@ -151,13 +131,102 @@ bool valid_camera(const rt_camera* c, std::string& why) {
         why = "camera: observer radius must be > 0";
         return false;
     }
+    if (static_cast<uint64_t>(c->width) * c->height > (1ull << 31)) {
+        why = "camera: more than 2^31 pixels";
+        return false;
+    }
     return true;
 }
 
-// Camera part: per-frame constants of the pixel loop and the observer's
-// Gram–Schmidt frame (core/kerr_schild_null_integrator.hpp:149-177), computed
-// once on the host because they are the same for every pixel.
-void fill_camera(const rt_camera* c, TraceParams& P) {
+// Metric, integrator and termination parts of the parameter block
+// (restating dispatch_fp64_scalar's set-up, render/software_compute_engine.hpp:3801-3809).
+void fill_params(const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm, TraceParams& P) {
+    std::memset(&P, 0, sizeof(P));
+    const bool multi = mt->n_sources >= 2;
+    if (multi) {
+        // central source (src[0]) sets the observer clamp and the budget rule
+        const double m0 = std::max(mt->src[0].mass, 1e-4);
+        P.M = m0;
+        P.a = std::clamp(mt->src[0].spin * m0, -0.999 * m0, 0.999 * m0);
+    } else {
+        P.M = std::max(mt->mass, 1e-4);
+        P.a = std::clamp(mt->spin, -0.999 * P.M, 0.999 * P.M);
+    }
+    P.a2 = P.a * P.a;
+    P.rh = horizon_radius(P.M, P.a);
+    P.r_hz = P.rh + tm->horizon_eps;
+    P.r_esc = tm->escape_radius;
+    P.disk_spin = std::clamp(P.a, -0.999 * P.M, 0.999 * P.M);
+    P.disk_on = tm->disk_enabled ? 1u : 0u;
+    P.clamp_g = (tm->flags & RT_TERM_NO_G_CLAMP) ? 0u : 1u;
+    const double isco = kerr_isco_host(P.M, P.a);
+    P.disk_in = (tm->disk_inner > 0.0) ? tm->disk_inner : isco;
+    P.disk_out = std::max((tm->disk_outer > 0.0) ? tm->disk_outer : 20.0 * P.M, P.disk_in * 1.05);
+    P.rh102 = P.rh * 1.02;
+    P.near_r = 3.0 * (2.0 * P.M);
+    P.n_src = multi ? mt->n_sources : 0u;
+    P.rtol = ig->rtol;
+    P.atol = ig->atol;
+    P.h0 = ig->h0;
+    P.h_min = ig->h_min;
+    P.h_max = ig->h_max;
+    P.inv_tol = ig->invariant_tol;
+    P.max_steps = ig->max_steps;
+    P.precision = ig->precision;
+    P.mixed = (!multi && ig->precision == RT_PRECISION_MIXED) ? 1u : 0u;
+    P.switch_r = std::max(ig->mixed_switch_radius, 3.0 * P.rh);
+}
+
+// Source table (restating build_multi_source_data, software_compute_engine.hpp:2738-2798):
+// source 0 keeps the world frame when its axis is +z; other sources get the
+// frame (helper x axis) x axis, axis x that; spins are a/m clamped to
+// +-0.9999 (+-0.999 for source 0).
+void fill_sources(const rt_metric* mt, const rt_termination* tm, MultiDev& T) {
+    std::memset(&T, 0, sizeof(T));
+    T.n = std::min<uint32_t>(mt->n_sources, RT_MAX_SOURCES);
+    for (uint32_t i = 0; i < T.n; ++i) {
+        const rt_source& b = mt->src[i];
+        SourceDev& s = T.src[i];
+        const double mb = (i == 0) ? std::max(b.mass, 1e-4) : std::max(b.mass, 1e-9);
+        const double ab =
+            (i == 0) ? std::clamp(b.spin * mb, -0.999 * mb, 0.999 * mb) : std::clamp(b.spin, -0.9999, 0.9999) * mb;
+        s.m = mb;
+        s.a = ab;
+        s.a2 = ab * ab;
+        s.rh = horizon_radius(mb, ab);
+        s.r_hz = s.rh + tm->horizon_eps;
+        s.rh102 = s.rh * 1.02;
+        for (int j = 0; j < 3; ++j) s.c[j] = b.position[j];
+        const double al = std::sqrt(b.axis[0] * b.axis[0] + b.axis[1] * b.axis[1] + b.axis[2] * b.axis[2]);
+        double ax[3] = {0.0, 0.0, 1.0};
+        if (al > 1e-12)
+            for (int j = 0; j < 3; ++j) ax[j] = b.axis[j] / al;
+        const double helper[3] = {std::abs(ax[2]) < 0.9 ? 0.0 : 1.0, 0.0, std::abs(ax[2]) < 0.9 ? 1.0 : 0.0};
+        double fx[3] = {helper[1] * ax[2] - helper[2] * ax[1], helper[2] * ax[0] - helper[0] * ax[2],
+                        helper[0] * ax[1] - helper[1] * ax[0]};
+        const double fl = std::sqrt(fx[0] * fx[0] + fx[1] * fx[1] + fx[2] * fx[2]);
+        for (int j = 0; j < 3; ++j) fx[j] /= fl;
+        const double fy[3] = {ax[1] * fx[2] - ax[2] * fx[1], ax[2] * fx[0] - ax[0] * fx[2],
+                              ax[0] * fx[1] - ax[1] * fx[0]};
+        const bool world = (i == 0 && al > 1e-12 && ax[0] == 0.0 && ax[1] == 0.0 && ax[2] > 0.0);
+        for (int j = 0; j < 3; ++j) {
+            s.ax[j] = ax[j];
+            s.fx[j] = world ? (j == 0 ? 1.0 : 0.0) : fx[j];
+            s.fy[j] = world ? (j == 1 ? 1.0 : 0.0) : fy[j];
+        }
+        s.disk_on = b.disk_enabled ? 1u : 0u;
+        const double din = (b.disk_inner > 0.0) ? b.disk_inner : kerr_isco_host(mb, ab);
+        s.disk_in = din;
+        s.disk_out = std::max((b.disk_outer > 0.0) ? b.disk_outer : 20.0 * mb, din * 1.05);
+    }
+}
+
+// Camera part: per-frame constants of the pixel loop and the observer frame,
+// the same for every pixel, so computed once here. Single Kerr: Gram–Schmidt
+// frame of kerr_schild_initial_momentum (core/kerr_schild_null_integrator.hpp:149-177).
+// Superposition: flat spherical observer position and the frame of
+// ms_initial_momentum (render/software_compute_engine.hpp:2919-2956).
+void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T) {
     const uint32_t first = std::min(c->row_offset, c->height);
     const uint32_t last = c->row_count ? std::min(first + c->row_count, c->height) : c->height;
     P.width = c->width;
@@ -174,17 +243,72 @@ void fill_camera(const rt_camera* c, TraceParams& P) {
         P.up[i] = c->up[i];
     }
     const double r_obs = std::max(c->observer[1], P.rh * 1.02);
-    const double th = std::clamp(c->observer[2], 0.001, 3.14159265358979323846 - 0.001);
+    const double th = std::clamp(c->observer[2], 0.001, kPiH - 0.001);
     const double ph = c->observer[3];
     P.sth = std::sin(th);
     P.cth = std::cos(th);
     P.sph = std::sin(ph);
     P.cph = std::cos(ph);
-    if (P.n_src >= 2) {
+    if (P.n_src >= 2 && T) {
         P.X0[0] = r_obs * P.sth * P.cph;
         P.X0[1] = r_obs * P.sth * P.sph;
         P.X0[2] = r_obs * P.cth;
-        return;  // superposition frame is filled by fill_sources()
+        const uint32_t n = T->n;
+        T->observer_inside = 0;
+        double psum = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const SourceDev& s = T->src[i];
+            double l[3];
+            to_local_host(s, P.X0, l);
+            const double diff = (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]) - s.a2;
+            const double r2 = std::max(0.5 * (diff + std::sqrt(diff * diff + 4.0 * s.a2 * l[2] * l[2])), 1e-24);
+            const double r = std::sqrt(r2);
+            if (r <= s.rh * 1.0001 || (r2 - 2.0 * s.m * r + s.a2) <= 0.0) T->observer_inside = 1;
+            const HField f = ks_field_host(l, s.m, s.a);
+            T->pot[i] = f.H;
+            for (int j = 0; j < 3; ++j) T->dir[i][j] = f.k[0] * s.fx[j] + f.k[1] * s.fy[j] + f.k[2] * s.ax[j];
+            psum += f.H;
+        }
+        auto mdot = [&](const double* A, const double* B) {
+            double v = -A[0] * B[0] + A[1] * B[1] + A[2] * B[2] + A[3] * B[3];
+            for (uint32_t i = 0; i < n; ++i) {
+                const double la = A[0] + T->dir[i][0] * A[1] + T->dir[i][1] * A[2] + T->dir[i][2] * A[3];
+                const double lb = B[0] + T->dir[i][0] * B[1] + T->dir[i][1] * B[2] + T->dir[i][2] * B[3];
+                v += 2.0 * T->pot[i] * la * lb;
+            }
+            return v;
+        };
+        const double ils = 1.0 / std::max(1.0 - 2.0 * psum, 1e-6);
+        auto lift = [&](double vx, double vy, double vz, double out[4]) {
+            double shift = 0.0;
+            for (uint32_t i = 0; i < n; ++i)
+                shift += T->pot[i] * (T->dir[i][0] * vx + T->dir[i][1] * vy + T->dir[i][2] * vz);
+            out[0] = 2.0 * shift * ils;
+            out[1] = vx;
+            out[2] = vy;
+            out[3] = vz;
+        };
+        double e1[4], e2[4], e3[4];
+        lift(1, 0, 0, e1);
+        lift(0, 1, 0, e2);
+        lift(0, 0, 1, e3);
+        const double n1 = std::sqrt(std::max(mdot(e1, e1), 1e-30));
+        for (int j = 0; j < 4; ++j) e1[j] /= n1;
+        const double e21 = mdot(e2, e1);
+        for (int j = 0; j < 4; ++j) e2[j] -= e21 * e1[j];
+        const double n2 = std::sqrt(std::max(mdot(e2, e2), 1e-30));
+        for (int j = 0; j < 4; ++j) e2[j] /= n2;
+        const double e31 = mdot(e3, e1), e32 = mdot(e3, e2);
+        for (int j = 0; j < 4; ++j) e3[j] -= e31 * e1[j] + e32 * e2[j];
+        const double n3 = std::sqrt(std::max(mdot(e3, e3), 1e-30));
+        for (int j = 0; j < 4; ++j) e3[j] /= n3;
+        T->u0 = std::sqrt(ils);
+        for (int j = 0; j < 4; ++j) {
+            T->e[0][j] = e1[j];
+            T->e[1][j] = e2[j];
+            T->e[2][j] = e3[j];
+        }
+        return;
     }
     // kerr_schild_position_from_boyer_lindquist (core/kerr_schild_null_integrator.hpp:46-54)
     const double st = std::sin(th);
@@ -223,68 +347,58 @@ void fill_camera(const rt_camera* c, TraceParams& P) {
 }
 
 // ------------------------------------------------------------ buffers
-DeviceResults::~DeviceResults() { release(); }
+DeviceArena::~DeviceArena() { release(); }
 
-void DeviceResults::release() {
+void DeviceArena::release() {
     if (base) (void)hipFree(base);
     base = nullptr;
-    capacity = 0;
-    std::memset(&ptrs, 0, sizeof(ptrs));
+    bytes = 0;
 }
 
-size_t DeviceResults::bytes_per_ray() { return 4 + 8 * 4 + 8 * 4 + 8 * 3 + 8 + 8 + 4 + 4 + 2 * 8 * 4; }
-
-rt_status DeviceResults::ensure(size_t n) {
-    if (n <= capacity && base) return RT_OK;
+rt_status DeviceArena::ensure(size_t need) {
+    if (need <= bytes && base) return RT_OK;
     release();
-    const size_t cap = std::max<size_t>(n, 1024);
-    // one allocation, 256-byte aligned sub-arrays
-    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
-    size_t off = 0;
-    const size_t o_cls = off;
-    off = align(off + cap * 4);
-    const size_t o_x = off;
-    off = align(off + cap * 32);
-    const size_t o_p = off;
-    off = align(off + cap * 32);
-    const size_t o_bl = off;
-    off = align(off + cap * 24);
-    const size_t o_lam = off;
-    off = align(off + cap * 8);
-    const size_t o_g = off;
-    off = align(off + cap * 8);
-    const size_t o_st = off;
-    off = align(off + cap * 4);
-    const size_t o_rj = off;
-    off = align(off + cap * 4);
-    const size_t o_ix = off;
-    off = align(off + cap * 32);
-    const size_t o_ip = off;
-    off = align(off + cap * 32);
-    const size_t o_cnt = off;
-    off = align(off + 16 * 8);
-    hipError_t e = hipMalloc(&base, off);
+    const size_t cap = std::max<size_t>(need, 1 << 16);
+    hipError_t e = hipMalloc(&base, cap);
     if (e != hipSuccess) {
         base = nullptr;
-        return hip_fail(e, "hipMalloc(results)");
+        return hip_fail(e, "hipMalloc");
     }
-    char* b = static_cast<char*>(base);
-    ptrs.cls = reinterpret_cast<uint32_t*>(b + o_cls);
-    ptrs.x = reinterpret_cast<double*>(b + o_x);
-    ptrs.p = reinterpret_cast<double*>(b + o_p);
-    ptrs.bl = reinterpret_cast<double*>(b + o_bl);
-    ptrs.lambda = reinterpret_cast<double*>(b + o_lam);
-    ptrs.g = reinterpret_cast<double*>(b + o_g);
-    ptrs.steps = reinterpret_cast<uint32_t*>(b + o_st);
-    ptrs.rejected = reinterpret_cast<uint32_t*>(b + o_rj);
-    ptrs.in_x = reinterpret_cast<double*>(b + o_ix);
-    ptrs.in_p = reinterpret_cast<double*>(b + o_ip);
-    ptrs.counters = reinterpret_cast<unsigned long long*>(b + o_cnt);
-    capacity = cap;
+    bytes = cap;
     return RT_OK;
 }
 
-// Copy n results with leading dimension ld (device) into the caller's arrays.
+static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+size_t results_bytes(size_t n, bool with_inputs) {
+    size_t b = align256(n * 4) + 2 * align256(n * 32) + align256(n * 24) + 2 * align256(n * 8) + 2 * align256(n * 4);
+    if (with_inputs) b += 2 * align256(n * 32);
+    return b;
+}
+
+OutPtrs carve_results(char*& cur, size_t n, bool with_inputs) {
+    OutPtrs o{};
+    auto take = [&](size_t bytes) {
+        char* p = cur;
+        cur += align256(bytes);
+        return p;
+    };
+    o.cls = reinterpret_cast<uint32_t*>(take(n * 4));
+    o.x = reinterpret_cast<double*>(take(n * 32));
+    o.p = reinterpret_cast<double*>(take(n * 32));
+    o.bl = reinterpret_cast<double*>(take(n * 24));
+    o.lambda = reinterpret_cast<double*>(take(n * 8));
+    o.g = reinterpret_cast<double*>(take(n * 8));
+    o.steps = reinterpret_cast<uint32_t*>(take(n * 4));
+    o.rejected = reinterpret_cast<uint32_t*>(take(n * 4));
+    if (with_inputs) {
+        o.in_x = reinterpret_cast<const double*>(take(n * 32));
+        o.in_p = reinterpret_cast<const double*>(take(n * 32));
+    }
+    o.n = n;
+    return o;
+}
+
 rt_status copy_results_to_host(const OutPtrs& d, size_t n, const rt_results* out, hipStream_t s) {
     if (!out) return fail(RT_ERR_INVALID_ARG, "null results");
     const size_t ld = d.n;
@@ -309,55 +423,79 @@ rt_status copy_results_to_host(const OutPtrs& d, size_t n, const rt_results* out
 
 }  // namespace rt
 
-// ===================================================================== context
-struct rt_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    rt::DeviceResults res;
-    size_t n_last = 0;
-    rt_stats stats{};
-    rt::LaunchShape shape;
-    std::mutex mu;
-};
-
 using namespace rt;
 
 namespace {
 
-rt_status begin(rt_context* ctx) {
-    RT_HIP(hipSetDevice(ctx->device));
-    return RT_OK;
-}
-
-// Launch one trace on the context's stream, timing it with HIP events.
-// count rays are traced; ld is the leading dimension of the SoA arrays.
-template <class Launch>
-rt_status run_trace(rt_context* ctx, const TraceParams& P, size_t count, size_t ld, Launch&& launch) {
+// Trace one request (camera or explicit rays) on the context's stream into
+// the context arena; times the kernel with HIP events.
+// x_soa/p_soa non-null: explicit rays (n of them), else camera cam.
+rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, const double* p_soa, size_t n_in,
+                    const rt_metric* metric, const rt_integrator* integ, const rt_termination* term) {
     const auto t0 = std::chrono::steady_clock::now();
-    rt_status st = ctx->res.ensure(ld);
+    RT_HIP(hipSetDevice(ctx->device));
+    TraceParams P;
+    fill_params(metric, integ, term, P);
+    const bool multi = P.n_src >= 2;
+    MultiDev T;
+    if (multi) fill_sources(metric, term, T);
+    size_t n;
+    if (cam) {
+        fill_camera(cam, P, multi ? &T : nullptr);
+        n = P.n_rays;
+    } else {
+        P.mode = 1;
+        P.n_rays = static_cast<uint32_t>(n_in);
+        n = n_in;
+        if (multi) {
+            T.observer_inside = 0;
+            T.u0 = 1.0;
+        }
+    }
+    if (n == 0) return fail(RT_ERR_INVALID_ARG, "no rays to trace");
+    const bool inputs = (cam == nullptr);
+    rt_status st = ctx->arena.ensure(results_bytes(n, inputs));
     if (st != RT_OK) return st;
-    OutPtrs O = ctx->res.ptrs;
-    O.n = ld;
-    RT_HIP(hipMemsetAsync(O.counters, 0, 16 * 8, ctx->stream));
+    st = ctx->aux.ensure(sizeof(MultiDev) + 256);
+    if (st != RT_OK) return st;
+    char* cur = static_cast<char*>(ctx->arena.base);
+    OutPtrs O = carve_results(cur, n, inputs);
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->aux.base);
+    MultiDev* Td = reinterpret_cast<MultiDev*>(static_cast<char*>(ctx->aux.base) + 256);
+    O.counters = counters;
+    if (inputs) {
+        for (int c = 0; c < 4; ++c) {
+            RT_HIP(hipMemcpyAsync(const_cast<double*>(O.in_x) + c * n, x_soa + c * n, n * 8, hipMemcpyHostToDevice,
+                                  ctx->stream));
+            RT_HIP(hipMemcpyAsync(const_cast<double*>(O.in_p) + c * n, p_soa + c * n, n * 8, hipMemcpyHostToDevice,
+                                  ctx->stream));
+        }
+    }
+    if (multi) RT_HIP(hipMemcpyAsync(Td, &T, sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    RT_HIP(launch_zero_u64(counters, 16, ctx->stream));
     RT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    RT_HIP(launch(P, O));
+    if (multi)
+        RT_HIP(launch_trace_multi(P, O, Td, ctx->stream));
+    else if (cam)
+        RT_HIP(launch_trace_camera(P, O, ctx->shape, ctx->stream));
+    else
+        RT_HIP(launch_trace_rays(P, O, ctx->shape, ctx->stream));
     RT_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     RT_HIP(hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;
     RT_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     unsigned long long cnt[5] = {0, 0, 0, 0, 0};
-    RT_HIP(hipMemcpy(cnt, O.counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    ctx->n_last = count;
-    ctx->res.ptrs.n = ld;
+    RT_HIP(hipMemcpy(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    ctx->last = O;
+    ctx->n_last = n;
     rt_stats& s = ctx->stats;
     s.kernel_ms = ms;
-    s.rays = count;
+    s.rays = n;
     s.accepted_steps = cnt[0];
     s.rejected_steps = cnt[1];
     s.rhs_evals = cnt[2];
     s.fp32_steps = cnt[3];
-    s.block_size = ctx->shape.block;
+    s.block_size = multi ? 256 : ctx->shape.block;
     s.grid_size = 0;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RT_OK;
@@ -422,8 +560,8 @@ void rt_kerr_metric(rt_metric* m, double mass, double spin) {
 }
 
 uint32_t rt_abi_version(void) { return RT_ABI_VERSION; }
-const char* rt_version(void) { return "librtsim 0.1 (gfx950, DP45 Kerr-Schild null tracer)"; }
-const char* rt_last_error(void) { return rt::g_last_error.c_str(); }
+const char* rt_version(void) { return "librtsim 0.2 (gfx950, DP45 Kerr-Schild null tracer)"; }
+const char* rt_last_error(void) { return rt::last_error_text(); }
 
 rt_status rt_context_create(int device, rt_context** out) {
     if (!out) return fail(RT_ERR_INVALID_ARG, "null out");
@@ -437,6 +575,7 @@ rt_status rt_context_create(int device, rt_context** out) {
     // Launch-shape override for experiments: RTSIM_BLOCK (64..256), RTSIM_WAVES (1|2).
     if (const char* b = std::getenv("RTSIM_BLOCK")) ctx->shape.block = static_cast<uint32_t>(std::atoi(b));
     if (const char* w = std::getenv("RTSIM_WAVES")) ctx->shape.waves = static_cast<uint32_t>(std::atoi(w));
+    ctx->qshape = ctx->shape;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
@@ -453,7 +592,9 @@ rt_status rt_context_destroy(rt_context* ctx) {
     if (!ctx) return RT_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    ctx->res.release();
+    rt::forget_graphs(ctx);
+    ctx->arena.release();
+    ctx->aux.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -472,28 +613,19 @@ rt_status rt_trace_camera_resident(rt_context* ctx, const rt_camera* cam, const 
     if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
     std::string why;
     if (!valid_request(metric, integ, term, why) || !valid_camera(cam, why)) return fail(RT_ERR_INVALID_ARG, why);
-    if (metric->n_sources >= 2) return fail(RT_ERR_INVALID_ARG, "superposition metrics are traced by rt_trace_batch");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    rt_status st = begin(ctx);
+    rt_status st = trace_one(ctx, cam, nullptr, nullptr, 0, metric, integ, term);
     if (st != RT_OK) return st;
-    TraceParams P;
-    fill_params(metric, integ, term, P);
-    fill_camera(cam, P);
-    const size_t n = P.n_rays;
-    st = run_trace(ctx, P, n, n, [&](const TraceParams& p, const OutPtrs& o) {
-        return launch_trace_camera(p, o, ctx->shape, ctx->stream);
-    });
-    if (st != RT_OK) return st;
-    if (n_rays) *n_rays = n;
+    if (n_rays) *n_rays = ctx->n_last;
     return RT_OK;
 }
 
 rt_status rt_fetch_results(rt_context* ctx, rt_results* out) {
     if (!ctx || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    rt_status st = begin(ctx);
-    if (st != RT_OK) return st;
-    return copy_results_to_host(ctx->res.ptrs, ctx->n_last, out, ctx->stream);
+    if (ctx->n_last == 0) return fail(RT_ERR_INVALID_ARG, "no resident results");
+    RT_HIP(hipSetDevice(ctx->device));
+    return copy_results_to_host(ctx->last, ctx->n_last, out, ctx->stream);
 }
 
 rt_status rt_trace_camera(rt_context* ctx, const rt_camera* cam, const rt_metric* metric, const rt_integrator* integ,
@@ -508,33 +640,13 @@ rt_status rt_trace_camera(rt_context* ctx, const rt_camera* cam, const rt_metric
 rt_status rt_trace_rays(rt_context* ctx, const double* x_soa, const double* p_soa, size_t n, const rt_metric* metric,
                         const rt_integrator* integ, const rt_termination* term, rt_results* out) {
     if (!ctx || !x_soa || !p_soa || !out || n == 0) return fail(RT_ERR_INVALID_ARG, "null argument or n == 0");
+    if (n > (1ull << 31)) return fail(RT_ERR_INVALID_ARG, "more than 2^31 rays");
     std::string why;
     if (!valid_request(metric, integ, term, why)) return fail(RT_ERR_INVALID_ARG, why);
-    if (metric->n_sources >= 2) return fail(RT_ERR_INVALID_ARG, "superposition metrics are traced by rt_trace_batch");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    rt_status st = begin(ctx);
+    rt_status st = trace_one(ctx, nullptr, x_soa, p_soa, n, metric, integ, term);
     if (st != RT_OK) return st;
-    st = ctx->res.ensure(n);
-    if (st != RT_OK) return st;
-    OutPtrs O = ctx->res.ptrs;
-    for (int c = 0; c < 4; ++c) {
-        RT_HIP(hipMemcpyAsync(const_cast<double*>(O.in_x) + c * ctx->res.capacity, x_soa + c * n, n * 8,
-                              hipMemcpyHostToDevice, ctx->stream));
-        RT_HIP(hipMemcpyAsync(const_cast<double*>(O.in_p) + c * ctx->res.capacity, p_soa + c * n, n * 8,
-                              hipMemcpyHostToDevice, ctx->stream));
-    }
-    TraceParams P;
-    fill_params(metric, integ, term, P);
-    P.mode = 1;
-    P.n_rays = static_cast<uint32_t>(n);
-    // Inputs are staged with the buffer capacity as leading dimension, so trace
-    // with that leading dimension and copy back the first n of each row.
-    const size_t ld = ctx->res.capacity;
-    st = run_trace(ctx, P, n, ld, [&](const TraceParams& p, const OutPtrs& o) {
-        return launch_trace_rays(p, o, ctx->shape, ctx->stream);
-    });
-    if (st != RT_OK) return st;
-    return copy_results_to_host(ctx->res.ptrs, n, out, ctx->stream);
+    return copy_results_to_host(ctx->last, n, out, ctx->stream);
 }
 
 }  // extern "C"

@@ -7,16 +7,28 @@
 namespace rt {
 
 // Launch shape chosen by the autotuner: threads per block (multiple of 64,
-// at most 256) and the occupancy the kernel is compiled for (1 or 2 waves per
-// SIMD; 2 caps registers at 256 and spills in the once-per-ray event code).
+// at most 256), the occupancy the kernel is compiled for (1 or 2 waves per
+// SIMD; 2 caps registers at 256 and spills in the once-per-ray event code)
+// and, for the work-queue kernel, how many lanes of a wave must be idle
+// before it refills them.
 struct LaunchShape {
     uint32_t block = 256;
     uint32_t waves = 1;
+    uint32_t refill_min = 16;
 };
 
-// Camera frame: one lane per pixel, lanes of a wave on an 8x8 tile.
+// Camera frame (single Kerr): one lane per pixel, lanes of a wave on an 8x8 tile.
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
-// Explicit initial states (O.in_x / O.in_p).
+// Explicit initial states (O.in_x / O.in_p), single Kerr.
 hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
+// Superposition (camera or explicit rays per P.mode), table T in device memory.
+hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, hipStream_t stream);
+// Persistent work-queue kernel over many single-Kerr requests.
+// grid 0: computed from the occupancy (not allowed inside a stream capture).
+hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, uint32_t grid, hipStream_t stream);
+// Grid that fills the device at the kernel's occupancy.
+hipError_t queue_grid(const LaunchShape& L, bool mixed, uint32_t* grid);
+// Zero n 64-bit counters with a kernel (no memset node in captured graphs).
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
 
 }  // namespace rt

@@ -1,9 +1,16 @@
 // rt_kernels.hip — HIP kernels of the null-geodesic tracer for gfx950.
 //
-// One lane traces one ray to its end: adaptive DP45 on the Kerr–Schild
-// Hamiltonian (first-same-as-last), events located on the dense output
-// (horizon, equatorial disk, escape), results written once as SoA. Lanes of a
-// wave cover an 8x8 pixel tile so their step counts stay close.
+// One lane traces one ray: adaptive DP45 on the Kerr–Schild Hamiltonian
+// (first-same-as-last), events located on the dense output (horizon,
+// equatorial disk, escape), results written once as SoA.
+//
+//   k_trace_camera_kerr  one camera frame, one lane per pixel, the lanes of a
+//                        wave on an 8x8 pixel tile (similar step counts)
+//   k_trace_rays_kerr    explicit initial states
+//   k_trace_queue        persistent kernel over a device work queue of many
+//                        requests: a wave refills its idle lanes from the
+//                        queue with one atomic (ballot + prefix count)
+//   k_trace_multi        superposition of up to 16 sources, table in LDS
 #include <hip/hip_runtime.h>
 
 #include "rt_device.hpp"
@@ -11,31 +18,22 @@
 
 namespace rt {
 
-struct RayState {
-    double y[8];
-    double xi;  // L_z / E of the ray (disk redshift)
-};
-
-struct RayEnd {
-    uint32_t cls;
-    double y[8];
-    double lambda, g;
-    uint32_t acc, rej;
+// ------------------------------------------------------------ lane state
+struct Lane {
+    double y[8], k1[8];
+    double ah, lambda, r0, xi, g;
+    uint32_t acc, rej, flags, cls;
     uint32_t rhs, proj, f32;
 };
 
-// Initial state of camera pixel (px, py) for a single Kerr hole: pinhole ray
-// (camera_projections.hpp:27-41), world direction from the camera basis,
-// projection onto e_r/e_theta/e_phi and back (software_compute_engine.hpp:3851-3938,
-// 1634-1645), photon momentum -outward in the observer's Gram–Schmidt frame,
-// normalised to p_t = -1 (kerr_schild_null_integrator.hpp:179-193). The frame
-// itself is identical for every pixel and comes precomputed in P.
-__device__ __forceinline__ RayState camera_state_kerr(const TraceParams& P, uint32_t px, uint32_t py) {
+// Camera pixel -> world direction -> the reference's `outward` vector:
+// pinhole ray (camera_projections.hpp:27-41), camera basis, projection onto
+// e_r/e_theta/e_phi and back (software_compute_engine.hpp:3851-3938, 1634-1645).
+__device__ __forceinline__ void camera_outward(const TraceParams& P, uint32_t px, uint32_t py, double o[3]) {
     const double v = 1.0 - (static_cast<double>(py) + 0.5) / static_cast<double>(P.height) * 2.0;
     const double u = ((static_cast<double>(px) + 0.5) / static_cast<double>(P.width) * 2.0 - 1.0) * P.aspect;
     const double n1 = 1.0, n2 = v * P.tan_half, n3 = u * P.tan_half;
-    const double len = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
-    const double inv_len = 1.0 / len;
+    const double inv_len = 1.0 / sqrt(n1 * n1 + n2 * n2 + n3 * n3);
     const double l0 = n1 * inv_len, l1 = n2 * inv_len, l2 = n3 * inv_len;
     const double d0 = l0 * P.fwd[0] + l2 * P.right[0] + l1 * P.up[0];
     const double d1 = l0 * P.fwd[1] + l2 * P.right[1] + l1 * P.up[1];
@@ -44,38 +42,45 @@ __device__ __forceinline__ RayState camera_state_kerr(const TraceParams& P, uint
     const double n_r = d0 * (st * cp) + d1 * (st * sp) + d2 * ct;
     const double n_th = d0 * (ct * cp) + d1 * (ct * sp) + d2 * (-st);
     const double n_ph = d0 * (-sp) + d1 * cp;
-    const double o0 = n_r * st * cp + n_th * ct * cp - n_ph * sp;
-    const double o1 = n_r * st * sp + n_th * ct * sp + n_ph * cp;
-    const double o2 = n_r * ct - n_th * st;
+    o[0] = n_r * st * cp + n_th * ct * cp - n_ph * sp;
+    o[1] = n_r * st * sp + n_th * ct * sp + n_ph * cp;
+    o[2] = n_r * ct - n_th * st;
+}
+
+// Photon momentum -outward in the observer's Gram–Schmidt frame, normalised
+// to p_t = -1 (kerr_schild_null_integrator.hpp:179-193). The frame is the same
+// for every pixel and comes precomputed in P.
+__device__ __forceinline__ void camera_state_kerr(const TraceParams& P, uint32_t px, uint32_t py, double y[8],
+                                                  double* xi) {
+    double o[3];
+    camera_outward(P, px, py, o);
     double vv[4] = {P.u0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        vv[c] -= o0 * P.frame[0][c];
-        vv[c] -= o1 * P.frame[1][c];
-        vv[c] -= o2 * P.frame[2][c];
+        vv[c] -= o[0] * P.frame[0][c];
+        vv[c] -= o[1] * P.frame[1][c];
+        vv[c] -= o[2] * P.frame[2][c];
     }
     const double two_h = P.two_h_obs;
     const double lv = vv[0] + P.kobs[0] * vv[1] + P.kobs[1] * vv[2] + P.kobs[2] * vv[3];
     const double p_t = -vv[0] + two_h * lv;
     const double inv_e = 1.0 / fmax(-p_t, 1e-12);
-    RayState s;
-    s.y[0] = 0.0;
-    s.y[1] = P.X0[0];
-    s.y[2] = P.X0[1];
-    s.y[3] = P.X0[2];
-    s.y[4] = -1.0;
-    s.y[5] = (vv[1] + two_h * lv * P.kobs[0]) * inv_e;
-    s.y[6] = (vv[2] + two_h * lv * P.kobs[1]) * inv_e;
-    s.y[7] = (vv[3] + two_h * lv * P.kobs[2]) * inv_e;
-    s.xi = s.y[1] * s.y[6] - s.y[2] * s.y[5];
-    return s;
+    y[0] = 0.0;
+    y[1] = P.X0[0];
+    y[2] = P.X0[1];
+    y[3] = P.X0[2];
+    y[4] = -1.0;
+    y[5] = (vv[1] + two_h * lv * P.kobs[0]) * inv_e;
+    y[6] = (vv[2] + two_h * lv * P.kobs[1]) * inv_e;
+    y[7] = (vv[3] + two_h * lv * P.kobs[2]) * inv_e;
+    *xi = y[1] * y[6] - y[2] * y[5];
 }
 
 // Render-step heuristic (software_compute_engine.hpp:1677-1683 with the
 // GpuCameraPushConstants defaults) as the initial |h|.
-__device__ __forceinline__ double initial_step_kerr(const TraceParams& P, double r) {
-    const double scale = sqrt(r * fmax(r - P.rh, 0.02 * P.M));
-    const double blend = fmin(fmax((r - 20.0 * P.rh) / (80.0 * P.rh), 0.0), 1.0);
+__device__ __forceinline__ double initial_step(double r, double rh, double m) {
+    const double scale = sqrt(r * fmax(r - rh, 0.02 * m));
+    const double blend = fmin(fmax((r - 20.0 * rh) / (80.0 * rh), 0.0), 1.0);
     const double far = 1.0 + (7.8 - 1.0) * blend;
     return fmin(fmax(0.5 * scale * far, 0.004), 3.5 * 7.8);
 }
@@ -115,13 +120,13 @@ __device__ __forceinline__ Field dp45_stages(const double y[8], const double k1[
     return rhs_kerr<R>(y5, M, a, a2, k7);
 }
 
-// Event location inside an accepted step [y, y5] of size h: horizon
-// r = r_hz, equatorial disk z = 0 with r in [disk_in, disk_out], escape
-// r = r_esc while outgoing; earliest first. Fills the end state and returns
-// the class (0: no event).
+// Event location inside an accepted step [y, y5] of size h (single Kerr):
+// horizon r = r_hz, equatorial disk z = 0 with r in [disk_in, disk_out],
+// escape r = r_esc while outgoing; earliest first. Fills the end state and
+// returns the class (0: no event).
 __device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const double y[8], const double k1[8], double h,
-                                              double r0, double r1, bool hz, bool dk, bool es, double xi, bool f32,
-                                              double yend[8], double* s_out, double* g_out) {
+                                                 double r0, double r1, bool hz, bool dk, bool es, double xi, bool f32,
+                                                 double yend[8], double* s_out, double* g_out) {
     const double M = P.M, a = P.a, a2 = P.a2;
     double y5[8], ep[8], k7[8], dn[8];
     if (f32)
@@ -188,149 +193,157 @@ __device__ __forceinline__ double next_step_fast(double ah, double err, bool acc
     return ah * fmin(fmax(0.9 * factor, 0.1), 0.9);
 }
 
-// Trace one ray in a single Kerr spacetime. Mixed: steps that start beyond
-// the switch radius evaluate the right-hand side in FP32 (state, stage sums,
-// error test, projection and event location stay FP64).
-template <bool Mixed>
-__device__ RayEnd trace_kerr(const TraceParams& P, const RayState& s0) {
-    const double M = P.M, a = P.a, a2 = P.a2;
-    RayEnd o;
-    o.g = 0.0;
-    double y[8], k1[8];
+// Start a ray: first RHS (FSAL seed) and initial step.
+__device__ __forceinline__ void lane_begin(const TraceParams& P, const double y0[8], double xi, Lane& L) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) y[i] = s0.y[i];
-    const Field f0 = rhs_kerr(y, M, a, a2, k1);
-    uint32_t rhs_n = 1, proj_n = 0, f32_n = 0;
-    double r0 = f0.r;
-    double ah = (P.h0 > 0.0) ? P.h0 : initial_step_kerr(P, r0);
-    ah = fmin(fmax(ah, P.h_min), P.h_max);
-    double lambda = 0.0;
-    uint32_t acc = 0, rej = 0, flags = 0, cls = 0;
-    for (;;) {
-        if (acc + rej >= P.max_steps || ah < P.h_min) {
-            flags |= (acc + rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
-            cls = (r0 < P.near_r) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) o.y[i] = y[i];
-            break;
-        }
-        const double h = -ah;
-        double y5[8], ep[8], k7[8];
-        const bool f32 = Mixed && (r0 > P.switch_r);
-        const Field f7 = f32 ? dp45_stages<false, float>(y, k1, h, M, a, a2, y5, ep, k7, nullptr)
-                             : dp45_stages<false, double>(y, k1, h, M, a, a2, y5, ep, k7, nullptr);
-        rhs_n += 6;
-        f32_n += f32 ? 1u : 0u;
-        double err = 0.0;
-        bool finite = true;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            finite = finite && isfinite(y5[i]);
-            if (i == 4) continue;  // dp_t/dl = 0: no error in p_t
-            const double e = ah * fabs(ep[i] + dp::e7 * k7[i]);
-            const double sc = P.rtol * fmax(fabs(y[i]), fabs(y5[i])) + P.atol;
-            err = fmax(err, e * rcp64(sc));
-        }
-        if (!finite || !isfinite(err)) {
-            flags |= RT_FLAG_NONFINITE;
-            cls = RT_CLS_CAPTURED;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) o.y[i] = y[i];
-            break;
-        }
-        if (err <= 1.0) {
-            ++acc;
-            const double r1 = f7.r;
-            const double z0 = y[3], z1 = y5[3];
-            const bool hz = (r1 <= P.r_hz);
-            const bool dk = P.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0));
-            const bool es = (r1 >= P.r_esc) && (-(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0);
-            if (hz || dk || es) {
-                double s_ev = 1.0, g_ev = 0.0;
-                const uint32_t c = locate_event(P, y, k1, h, r0, r1, hz, dk, es, s0.xi, f32, o.y, &s_ev, &g_ev);
-                if (c != 0) {
-                    rhs_n += 6;
-                    const Field fe = ks_field(o.y[1], o.y[2], o.y[3], M, a, a2);
-                    if (project_pt(o.y, fe.H, fe.k0 * o.y[5] + fe.k1 * o.y[6] + fe.k2 * o.y[7], P.inv_tol)) {
-                        flags |= RT_FLAG_PROJECTED;
-                        ++proj_n;
-                    }
-                    lambda += s_ev * ah;
-                    cls = c;
-                    o.g = g_ev;
-                    break;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i) y[i] = y5[i];
-            lambda += ah;
-            if (project_pt(y, f7.H, f7.k0 * y[5] + f7.k1 * y[6] + f7.k2 * y[7], P.inv_tol)) {
-                flags |= RT_FLAG_PROJECTED;
-                ++proj_n;
-                if (f32)
-                    rhs_kerr<float>(y, M, a, a2, k1);
-                else
-                    rhs_kerr<double>(y, M, a, a2, k1);
-                ++rhs_n;
-            } else {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) k1[i] = k7[i];
-            }
-            r0 = r1;
-            ah = Mixed ? next_step_fast(ah, err, true, P.h_min, P.h_max) : next_step(ah, err, true, P.h_min, P.h_max);
-        } else {
-            ++rej;
-            ah = Mixed ? next_step_fast(ah, err, false, P.h_min, P.h_max)
-                       : next_step(ah, err, false, P.h_min, P.h_max);
-        }
-    }
-    const uint32_t c = cls & RT_CLS_MASK;
-    if (c == RT_CLS_ESCAPED) o.g = 1.0 / (-o.y[4]);
-    if (c == RT_CLS_CAPTURED) o.g = 0.0;
-    o.cls = cls | flags;
-    o.lambda = lambda;
-    o.acc = acc;
-    o.rej = rej;
-    o.rhs = rhs_n;
-    o.proj = proj_n;
-    o.f32 = f32_n;
-    if (f32_n) o.cls |= RT_FLAG_FP32_STEPS;
-    return o;
+    for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
+    const Field f0 = rhs_kerr<double>(L.y, P.M, P.a, P.a2, L.k1);
+    L.r0 = f0.r;
+    const double ah = (P.h0 > 0.0) ? P.h0 : initial_step(L.r0, P.rh, P.M);
+    L.ah = fmin(fmax(ah, P.h_min), P.h_max);
+    L.xi = xi;
+    L.lambda = 0.0;
+    L.g = 0.0;
+    L.acc = L.rej = L.flags = L.cls = 0;
+    L.rhs = 1;
+    L.proj = L.f32 = 0;
 }
 
-__device__ __forceinline__ void store_end(const TraceParams& P, const OutPtrs& O, uint64_t idx, const RayEnd& e) {
+// One DP45 attempt of a single-Kerr ray. Returns true when the ray ended:
+// L.y then holds the end state and L.cls its class. Mixed: an attempt that
+// starts beyond switch_r evaluates the RHS in FP32 (state, stage sums, error
+// test, projection and event location stay FP64).
+template <bool Mixed>
+__device__ __forceinline__ bool lane_step(const TraceParams& P, Lane& L) {
+    const double M = P.M, a = P.a, a2 = P.a2;
+    if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
+        L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
+        L.cls = (L.r0 < P.near_r) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
+        return true;
+    }
+    const double ah = L.ah;
+    const double h = -ah;
+    double y5[8], ep[8], k7[8];
+    const bool f32 = Mixed && P.mixed && (L.r0 > P.switch_r);
+    const Field f7 = f32 ? dp45_stages<false, float>(L.y, L.k1, h, M, a, a2, y5, ep, k7, nullptr)
+                         : dp45_stages<false, double>(L.y, L.k1, h, M, a, a2, y5, ep, k7, nullptr);
+    L.rhs += 6;
+    L.f32 += f32 ? 1u : 0u;
+    double err = 0.0;
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        finite = finite && isfinite(y5[i]);
+        if (i == 4) continue;  // dp_t/dl = 0: no error in p_t
+        const double e = ah * fabs(ep[i] + dp::e7 * k7[i]);
+        const double sc = P.rtol * fmax(fabs(L.y[i]), fabs(y5[i])) + P.atol;
+        err = fmax(err, e * rcp64(sc));
+    }
+    if (!finite || !isfinite(err)) {
+        L.flags |= RT_FLAG_NONFINITE;
+        L.cls = RT_CLS_CAPTURED;
+        return true;
+    }
+    const bool fast = Mixed && P.mixed;
+    if (!(err <= 1.0)) {
+        ++L.rej;
+        L.ah = fast ? next_step_fast(ah, err, false, P.h_min, P.h_max) : next_step(ah, err, false, P.h_min, P.h_max);
+        return false;
+    }
+    ++L.acc;
+    const double r1 = f7.r;
+    const double z0 = L.y[3], z1 = y5[3];
+    const bool hz = (r1 <= P.r_hz);
+    const bool dk = P.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0));
+    const bool es = (r1 >= P.r_esc) && (-(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0);
+    if (hz || dk || es) {
+        double s_ev = 1.0, g_ev = 0.0, ye[8];
+        const uint32_t c = locate_event(P, L.y, L.k1, h, L.r0, r1, hz, dk, es, L.xi, f32, ye, &s_ev, &g_ev);
+        if (c != 0) {
+            L.rhs += 6;
+            const Field fe = ks_field(ye[1], ye[2], ye[3], M, a, a2);
+            if (project_pt(ye, fe.H, fe.k0 * ye[5] + fe.k1 * ye[6] + fe.k2 * ye[7], P.inv_tol)) {
+                L.flags |= RT_FLAG_PROJECTED;
+                ++L.proj;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) L.y[i] = ye[i];
+            L.lambda += s_ev * ah;
+            L.cls = c;
+            L.g = g_ev;
+            return true;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
+    L.lambda += ah;
+    if (project_pt(L.y, f7.H, f7.k0 * L.y[5] + f7.k1 * L.y[6] + f7.k2 * L.y[7], P.inv_tol)) {
+        L.flags |= RT_FLAG_PROJECTED;
+        ++L.proj;
+        if (f32)
+            rhs_kerr<float>(L.y, M, a, a2, L.k1);
+        else
+            rhs_kerr<double>(L.y, M, a, a2, L.k1);
+        ++L.rhs;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) L.k1[i] = k7[i];
+    }
+    L.r0 = r1;
+    L.ah = fast ? next_step_fast(ah, err, true, P.h_min, P.h_max) : next_step(ah, err, true, P.h_min, P.h_max);
+    return false;
+}
+
+// Redshift per class and the final flag word.
+__device__ __forceinline__ void lane_finish(Lane& L) {
+    const uint32_t c = L.cls & RT_CLS_MASK;
+    if (c == RT_CLS_ESCAPED) L.g = 1.0 / (-L.y[4]);
+    if (c == RT_CLS_CAPTURED) L.g = 0.0;
+    L.cls |= L.flags;
+    if (L.f32) L.cls |= RT_FLAG_FP32_STEPS;
+}
+
+__device__ __forceinline__ void store_lane(const OutPtrs& O, uint64_t idx, const Lane& L, double a, double a2) {
     const uint64_t n = O.n;
-    O.cls[idx] = e.cls;
+    O.cls[idx] = L.cls;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        O.x[c * n + idx] = e.y[c];
-        O.p[c * n + idx] = e.y[4 + c];
+        O.x[c * n + idx] = L.y[c];
+        O.p[c * n + idx] = L.y[4 + c];
     }
     double bl[3];
-    bl_from_ks(e.y[1], e.y[2], e.y[3], P.a, P.a2, bl);
+    bl_from_ks(L.y[1], L.y[2], L.y[3], a, a2, bl);
 #pragma unroll
     for (int c = 0; c < 3; ++c) O.bl[c * n + idx] = bl[c];
-    O.lambda[idx] = e.lambda;
-    O.g[idx] = e.g;
-    O.steps[idx] = e.acc;
-    O.rejected[idx] = e.rej;
+    O.lambda[idx] = L.lambda;
+    O.g[idx] = L.g;
+    O.steps[idx] = L.acc;
+    O.rejected[idx] = L.rej;
 }
 
-__device__ __forceinline__ void add_counters(const OutPtrs& O, bool valid, const RayEnd& e) {
-    const unsigned long long acc = wave_sum(valid ? e.acc : 0u);
-    const unsigned long long rej = wave_sum(valid ? e.rej : 0u);
-    const unsigned long long rhs = wave_sum(valid ? e.rhs : 0u);
-    const unsigned long long proj = wave_sum(valid ? e.proj : 0u);
-    const unsigned long long f32 = wave_sum(valid ? e.f32 : 0u);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&O.counters[0], acc);
-        atomicAdd(&O.counters[1], rej);
-        atomicAdd(&O.counters[2], rhs);
-        atomicAdd(&O.counters[3], f32);
-        atomicAdd(&O.counters[4], proj);
+struct Tally {
+    unsigned long long acc = 0, rej = 0, rhs = 0, f32 = 0, proj = 0;
+    __device__ void add(const Lane& L) {
+        acc += L.acc;
+        rej += L.rej;
+        rhs += L.rhs;
+        f32 += L.f32;
+        proj += L.proj;
     }
-}
+    __device__ void flush(unsigned long long* counters) {
+        const unsigned long long sa = wave_sum(acc), sr = wave_sum(rej), sh = wave_sum(rhs), sf = wave_sum(f32),
+                                 sp = wave_sum(proj);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&counters[0], sa);
+            atomicAdd(&counters[1], sr);
+            atomicAdd(&counters[2], sh);
+            atomicAdd(&counters[3], sf);
+            atomicAdd(&counters[4], sp);
+        }
+    }
+};
 
+// ------------------------------------------------------- plain kernels
 // Camera frame, single Kerr. Thread t -> 8x8 tile t/64, lane t%64.
 template <bool Mixed, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P, OutPtrs O) {
@@ -340,37 +353,520 @@ __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P,
     const uint32_t px = (tile % P.tiles_x) * 8 + (lane & 7);
     const uint32_t py = (tile / P.tiles_x) * 8 + (lane >> 3);
     const uint32_t rows = P.n_rays / P.width;
-    const bool valid = (px < P.width) && (py < rows);
-    RayEnd e{};
-    if (valid) {
-        const RayState s = camera_state_kerr(P, px, P.row_first + py);
-        e = trace_kerr<Mixed>(P, s);
-        store_end(P, O, static_cast<uint64_t>(py) * P.width + px, e);
+    Tally tally;
+    if (px < P.width && py < rows) {
+        double y0[8], xi;
+        camera_state_kerr(P, px, P.row_first + py, y0, &xi);
+        Lane L;
+        lane_begin(P, y0, xi, L);
+        while (!lane_step<Mixed>(P, L)) {
+        }
+        lane_finish(L);
+        store_lane(O, static_cast<uint64_t>(py) * P.width + px, L, P.a, P.a2);
+        tally.add(L);
     }
-    add_counters(O, valid, e);
+    tally.flush(O.counters);
 }
 
 // Explicit initial states, single Kerr.
 template <bool Mixed, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_trace_rays_kerr(TraceParams P, OutPtrs O) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < P.n_rays;
-    RayEnd e{};
-    if (valid) {
-        RayState s;
+    Tally tally;
+    if (i < P.n_rays) {
+        double y0[8];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            s.y[c] = O.in_x[c * O.n + i];
-            s.y[4 + c] = O.in_p[c * O.n + i];
+            y0[c] = O.in_x[c * O.n + i];
+            y0[4 + c] = O.in_p[c * O.n + i];
         }
-        s.xi = (s.y[1] * s.y[6] - s.y[2] * s.y[5]) / (-s.y[4]);
-        e = trace_kerr<Mixed>(P, s);
-        store_end(P, O, i, e);
+        const double xi = (y0[1] * y0[6] - y0[2] * y0[5]) / (-y0[4]);
+        Lane L;
+        lane_begin(P, y0, xi, L);
+        while (!lane_step<Mixed>(P, L)) {
+        }
+        lane_finish(L);
+        store_lane(O, i, L, P.a, P.a2);
+        tally.add(L);
     }
-    add_counters(O, valid, e);
+    tally.flush(O.counters);
+}
+
+// ------------------------------------------------------ work-queue kernel
+// Persistent: every wave loops until the queue is drained and its lanes are
+// idle. Idle lanes are refilled together: one ballot, one atomicAdd on the
+// queue head for the whole wave, each idle lane taking base + (its rank among
+// the idle lanes). A refill happens once at least refill_min lanes are idle
+// (or none is active), which bounds the atomics and the divergent set-up code.
+// Each lane carries its own request; the request's parameters are read from
+// the device table (cached), so lanes of different requests share a wave.
+__device__ __forceinline__ uint32_t find_request(const QueueDev& Q, uint64_t g) {
+    uint32_t lo = 0, hi = Q.n_req;  // req_offset[lo] <= g < req_offset[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (Q.req_offset[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <bool Mixed, int Waves>
+__global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
+    const uint32_t lane_id = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane_id == 0) ? 0ull : (~0ull >> (64 - lane_id));
+    Lane L;
+    bool active = false;
+    bool drained = false;
+    uint32_t req = 0;
+    uint64_t ray = 0;
+    Tally tally;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t n_idle = __popcll(idle);
+        if (!drained && n_idle != 0 && (n_idle >= Q.refill_min || n_idle == 64)) {
+            const uint32_t leader = __ffsll(static_cast<unsigned long long>(idle)) - 1;
+            unsigned long long base = 0;
+            if (lane_id == leader) base = atomicAdd(Q.next, static_cast<unsigned long long>(n_idle));
+            base = __shfl(base, leader, 64);
+            if (base + n_idle >= Q.total) drained = true;
+            if (!active) {
+                const uint64_t g = base + __popcll(idle & lt_mask);
+                if (g < Q.total) {
+                    req = find_request(Q, g);
+                    ray = g - Q.req_offset[req];
+                    const TraceParams& P = Q.reqs[req];
+                    double y0[8], xi = 0.0;
+                    bool ok = true;
+                    if (P.mode == 0) {
+                        const uint32_t tile = static_cast<uint32_t>(ray >> 6), l = static_cast<uint32_t>(ray & 63);
+                        const uint32_t px = (tile % P.tiles_x) * 8 + (l & 7);
+                        const uint32_t py = (tile / P.tiles_x) * 8 + (l >> 3);
+                        ok = (px < P.width) && (py < P.n_rays / P.width);
+                        if (ok) {
+                            camera_state_kerr(P, px, P.row_first + py, y0, &xi);
+                            ray = static_cast<uint64_t>(py) * P.width + px;
+                        }
+                    } else {
+                        const OutPtrs& O = Q.outs[req];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            y0[c] = O.in_x[c * O.n + ray];
+                            y0[4 + c] = O.in_p[c * O.n + ray];
+                        }
+                        xi = (y0[1] * y0[6] - y0[2] * y0[5]) / (-y0[4]);
+                    }
+                    if (ok) {
+                        lane_begin(P, y0, xi, L);
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            const TraceParams& P = Q.reqs[req];
+            if (lane_step<Mixed>(P, L)) {
+                lane_finish(L);
+                store_lane(Q.outs[req], ray, L, P.a, P.a2);
+                tally.add(L);
+                active = false;
+            }
+        }
+    }
+    tally.flush(Q.counters);
+}
+
+// ------------------------------------------------------- superposition
+// Right-hand side of the superposed Hamiltonian (ms_rhs,
+// software_compute_engine.hpp:2857-2870, with a carried p_t): flat part plus
+// the rotated KS terms of every source, sources read from LDS.
+__device__ __forceinline__ void rhs_multi(const MultiDev& T, const double y[8], double d[8]) {
+    d[0] = -y[4];
+    d[1] = y[5];
+    d[2] = y[6];
+    d[3] = y[7];
+    d[4] = 0.0;
+    d[5] = d[6] = d[7] = 0.0;
+    for (uint32_t i = 0; i < T.n; ++i) {
+        const SourceDev& s = T.src[i];
+        const double dx = y[1] - s.c[0], dy = y[2] - s.c[1], dz = y[3] - s.c[2];
+        const double lx = dx * s.fx[0] + dy * s.fx[1] + dz * s.fx[2];
+        const double ly = dx * s.fy[0] + dy * s.fy[1] + dz * s.fy[2];
+        const double lz = dx * s.ax[0] + dy * s.ax[1] + dz * s.ax[2];
+        const double px = y[5] * s.fx[0] + y[6] * s.fx[1] + y[7] * s.fx[2];
+        const double py = y[5] * s.fy[0] + y[6] * s.fy[1] + y[7] * s.fy[2];
+        const double pz = y[5] * s.ax[0] + y[6] * s.ax[1] + y[7] * s.ax[2];
+        double ths, c[3], q[3];
+        ks_terms<double>(lx, ly, lz, y[4], px, py, pz, s.m, s.a, s.a2, ths, c, q);
+        d[0] += ths;
+        d[1] += c[0] * s.fx[0] + c[1] * s.fy[0] + c[2] * s.ax[0];
+        d[2] += c[0] * s.fx[1] + c[1] * s.fy[1] + c[2] * s.ax[1];
+        d[3] += c[0] * s.fx[2] + c[1] * s.fy[2] + c[2] * s.ax[2];
+        d[5] += q[0] * s.fx[0] + q[1] * s.fy[0] + q[2] * s.ax[0];
+        d[6] += q[0] * s.fx[1] + q[1] * s.fy[1] + q[2] * s.ax[1];
+        d[7] += q[0] * s.fx[2] + q[1] * s.fy[2] + q[2] * s.ax[2];
+    }
+}
+
+__device__ __forceinline__ void to_local(const SourceDev& s, double x, double y, double z, double l[3]) {
+    const double dx = x - s.c[0], dy = y - s.c[1], dz = z - s.c[2];
+    l[0] = dx * s.fx[0] + dy * s.fx[1] + dz * s.fx[2];
+    l[1] = dx * s.fy[0] + dy * s.fy[1] + dz * s.fy[2];
+    l[2] = dx * s.ax[0] + dy * s.ax[1] + dz * s.ax[2];
+}
+
+// Superposed null constraint as a quadratic in p_t; projection as for one hole.
+__device__ __forceinline__ bool project_multi(const MultiDev& T, double y[8], double tol) {
+    double sh = 0.0, shq = 0.0, shq2 = 0.0;
+    for (uint32_t i = 0; i < T.n; ++i) {
+        const SourceDev& s = T.src[i];
+        double l[3];
+        to_local(s, y[1], y[2], y[3], l);
+        const double px = y[5] * s.fx[0] + y[6] * s.fx[1] + y[7] * s.fx[2];
+        const double py = y[5] * s.fy[0] + y[6] * s.fy[1] + y[7] * s.fy[2];
+        const double pz = y[5] * s.ax[0] + y[6] * s.ax[1] + y[7] * s.ax[2];
+        const Field f = ks_field(l[0], l[1], l[2], s.m, s.a, s.a2);
+        const double q = f.k0 * px + f.k1 * py + f.k2 * pz;
+        sh += f.H;
+        shq += f.H * q;
+        shq2 += f.H * q * q;
+    }
+    const double P2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
+    const double A = -(1.0 + 2.0 * sh);
+    const double B = 4.0 * shq;
+    const double C0 = P2 - 2.0 * shq2;
+    const double res = (A * y[4] + B) * y[4] + C0;
+    const double scale = fmax(1.0, y[4] * y[4] + P2);
+    if (!(fabs(res) >= tol * scale)) return false;
+    const double disc = B * B - 4.0 * A * C0;
+    if (disc < 0.0) return false;
+    const double sd = sqrt(disc);
+    const double r1 = (-B + sd) / (2.0 * A);
+    const double r2 = (-B - sd) / (2.0 * A);
+    y[4] = (fabs(r1 - y[4]) <= fabs(r2 - y[4])) ? r1 : r2;
+    return true;
+}
+
+// Camera pixel -> initial state in a superposition (trace_multi_source set-up,
+// software_compute_engine.hpp:3054-3066, and ms_initial_momentum :2919-2964):
+// flat spherical observer position, superposed-metric frame precomputed.
+__device__ __forceinline__ void camera_state_multi(const TraceParams& P, const MultiDev& T, uint32_t px, uint32_t py,
+                                                   double y[8]) {
+    double o[3];
+    camera_outward(P, px, py, o);
+    double vv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vv[c] = (c == 0 ? T.u0 : 0.0) - o[0] * T.e[0][c] - o[1] * T.e[1][c] - o[2] * T.e[2][c];
+    double mom0 = vv[1], mom1 = vv[2], mom2 = vv[3];
+    double tc = -vv[0];
+    for (uint32_t i = 0; i < T.n; ++i) {
+        const double lv = vv[0] + T.dir[i][0] * vv[1] + T.dir[i][1] * vv[2] + T.dir[i][2] * vv[3];
+        const double w = 2.0 * T.pot[i] * lv;
+        tc += w;
+        mom0 += w * T.dir[i][0];
+        mom1 += w * T.dir[i][1];
+        mom2 += w * T.dir[i][2];
+    }
+    const double inv_pt = 1.0 / fmax(-tc, 1e-12);
+    y[0] = 0.0;
+    y[1] = P.X0[0];
+    y[2] = P.X0[1];
+    y[3] = P.X0[2];
+    y[4] = -1.0;
+    y[5] = mom0 * inv_pt;
+    y[6] = mom1 * inv_pt;
+    y[7] = mom2 * inv_pt;
+}
+
+// Trace one ray through a superposition (FP64): DP45 as for one hole; events
+// per source (horizon of its local KS radius, crossing of its local equatorial
+// plane inside its disk), escape on the Euclidean radius while outgoing.
+__device__ void trace_multi(const TraceParams& P, const MultiDev& T, const double y0[8], Lane& L) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
+    rhs_multi(T, L.y, L.k1);
+    L.rhs = 1;
+    L.proj = L.f32 = 0;
+    L.acc = L.rej = L.flags = L.cls = 0;
+    L.lambda = 0.0;
+    L.g = 0.0;
+    {
+        double scale = 1e300, blend = 1.0;
+        for (uint32_t i = 0; i < T.n; ++i) {
+            const SourceDev& s = T.src[i];
+            double l[3];
+            to_local(s, L.y[1], L.y[2], L.y[3], l);
+            const double r = ks_radius(l[0], l[1], l[2], s.a2);
+            scale = fmin(scale, sqrt(r * fmax(r - s.rh, 0.02 * s.m)));
+            blend = fmin(blend, fmin(fmax((r - 20.0 * s.rh) / (80.0 * s.rh), 0.0), 1.0));
+        }
+        const double ah = (P.h0 > 0.0) ? P.h0 : fmin(fmax(0.5 * scale * (1.0 + 6.8 * blend), 0.004), 3.5 * 7.8);
+        L.ah = fmin(fmax(ah, P.h_min), P.h_max);
+    }
+    for (;;) {
+        if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
+            L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
+            bool near = false;
+            for (uint32_t i = 0; i < T.n; ++i) {
+                const SourceDev& s = T.src[i];
+                const double dx = L.y[1] - s.c[0], dy = L.y[2] - s.c[1], dz = L.y[3] - s.c[2];
+                near = near || (sqrt(dx * dx + dy * dy + dz * dz) < 6.0 * s.m);
+            }
+            L.cls = near ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
+            return;
+        }
+        const double ah = L.ah, h = -ah;
+        double k2[8], k3[8], k4[8], k5[8], k6[8], k7[8], Y[8], y5[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Y[i] = L.y[i] + h * (dp::a21 * L.k1[i]);
+        rhs_multi(T, Y, k2);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Y[i] = L.y[i] + h * (dp::a31 * L.k1[i] + dp::a32 * k2[i]);
+        rhs_multi(T, Y, k3);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Y[i] = L.y[i] + h * (dp::a41 * L.k1[i] + dp::a42 * k2[i] + dp::a43 * k3[i]);
+        rhs_multi(T, Y, k4);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            Y[i] = L.y[i] + h * (dp::a51 * L.k1[i] + dp::a52 * k2[i] + dp::a53 * k3[i] + dp::a54 * k4[i]);
+        rhs_multi(T, Y, k5);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            Y[i] = L.y[i] + h * (dp::a61 * L.k1[i] + dp::a62 * k2[i] + dp::a63 * k3[i] + dp::a64 * k4[i] +
+                                 dp::a65 * k5[i]);
+        rhs_multi(T, Y, k6);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            y5[i] = L.y[i] + h * (dp::a71 * L.k1[i] + dp::a73 * k3[i] + dp::a74 * k4[i] + dp::a75 * k5[i] +
+                                  dp::a76 * k6[i]);
+        rhs_multi(T, y5, k7);
+        L.rhs += 6;
+        double err = 0.0;
+        bool finite = true;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            finite = finite && isfinite(y5[i]);
+            if (i == 4) continue;
+            const double e = ah * fabs(dp::e1 * L.k1[i] + dp::e3 * k3[i] + dp::e4 * k4[i] + dp::e5 * k5[i] +
+                                       dp::e6 * k6[i] + dp::e7 * k7[i]);
+            const double sc = P.rtol * fmax(fabs(L.y[i]), fabs(y5[i])) + P.atol;
+            err = fmax(err, e * rcp64(sc));
+        }
+        if (!finite || !isfinite(err)) {
+            L.flags |= RT_FLAG_NONFINITE;
+            L.cls = RT_CLS_CAPTURED;
+            return;
+        }
+        if (!(err <= 1.0)) {
+            ++L.rej;
+            L.ah = next_step(ah, err, false, P.h_min, P.h_max);
+            continue;
+        }
+        ++L.acc;
+        Dense q[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            q[i].D = y5[i] - L.y[i];
+            q[i].B = h * L.k1[i] - q[i].D;
+            q[i].C = q[i].D - h * k7[i] - q[i].B;
+            q[i].E = h * (dp::d1 * L.k1[i] + dp::d3 * k3[i] + dp::d4 * k4[i] + dp::d5 * k5[i] + dp::d6 * k6[i] +
+                          dp::d7 * k7[i]);
+        }
+        auto xyz_at = [&](double ss, double out[3]) {
+            out[0] = dense_at(q[1], L.y[1], ss);
+            out[1] = dense_at(q[2], L.y[2], ss);
+            out[2] = dense_at(q[3], L.y[3], ss);
+        };
+        double best = 2.0, best_g = 0.0;
+        uint32_t best_cls = 0;
+        for (uint32_t i = 0; i < T.n; ++i) {
+            const SourceDev& s = T.src[i];
+            double l0[3], l1[3];
+            to_local(s, L.y[1], L.y[2], L.y[3], l0);
+            to_local(s, y5[1], y5[2], y5[3], l1);
+            const double r1 = ks_radius(l1[0], l1[1], l1[2], s.a2);
+            if (r1 <= s.r_hz) {
+                const double r0 = ks_radius(l0[0], l0[1], l0[2], s.a2);
+                if (r0 > s.r_hz) {
+                    const double sh = illinois_root(
+                        [&](double ss) {
+                            double x[3], l[3];
+                            xyz_at(ss, x);
+                            to_local(s, x[0], x[1], x[2], l);
+                            return ks_radius(l[0], l[1], l[2], s.a2) - s.r_hz;
+                        },
+                        r0 - s.r_hz, r1 - s.r_hz);
+                    if (sh < best) {
+                        best = sh;
+                        best_cls = RT_CLS_CAPTURED;
+                    }
+                }
+            }
+            const double z0 = l0[2], z1 = l1[2];
+            if (s.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0))) {
+                const double sd = illinois_root(
+                    [&](double ss) {
+                        double x[3], l[3];
+                        xyz_at(ss, x);
+                        to_local(s, x[0], x[1], x[2], l);
+                        return l[2];
+                    },
+                    z0, z1);
+                double x[3], l[3];
+                xyz_at(sd, x);
+                to_local(s, x[0], x[1], x[2], l);
+                const double rc = ks_radius(l[0], l[1], l[2], s.a2);
+                if (rc >= s.disk_in && rc <= s.disk_out && rc > s.rh102 && sd < best) {
+                    const double pw0 = dense_at(q[5], L.y[5], sd), pw1 = dense_at(q[6], L.y[6], sd),
+                                 pw2 = dense_at(q[7], L.y[7], sd);
+                    const double pl0 = pw0 * s.fx[0] + pw1 * s.fx[1] + pw2 * s.fx[2];
+                    const double pl1 = pw0 * s.fy[0] + pw1 * s.fy[1] + pw2 * s.fy[2];
+                    const double pl2 = pw0 * s.ax[0] + pw1 * s.ax[1] + pw2 * s.ax[2];
+                    const double pm = sqrt(pl0 * pl0 + pl1 * pl1 + pl2 * pl2);
+                    const double lzv = (l[0] * pl1 - l[1] * pl0) / fmax(pm, 1e-30);
+                    best = sd;
+                    best_cls = RT_CLS_DISK;
+                    best_g = disk_redshift(s.m, s.a, rc, lzv, P.clamp_g != 0);
+                }
+            }
+        }
+        {
+            const double R1 = sqrt(y5[1] * y5[1] + y5[2] * y5[2] + y5[3] * y5[3]);
+            if (R1 >= P.r_esc && -(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0) {
+                const double R0 = sqrt(L.y[1] * L.y[1] + L.y[2] * L.y[2] + L.y[3] * L.y[3]);
+                double se = 1.0;
+                if (R0 < P.r_esc)
+                    se = illinois_root(
+                        [&](double ss) {
+                            double x[3];
+                            xyz_at(ss, x);
+                            return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) - P.r_esc;
+                        },
+                        R0 - P.r_esc, R1 - P.r_esc);
+                if (se < best) {
+                    best = se;
+                    best_cls = RT_CLS_ESCAPED;
+                }
+            }
+        }
+        if (best_cls != 0) {
+            if (best >= 1.0) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) L.y[i] = dense_at(q[i], L.y[i], best);
+            }
+            if (project_multi(T, L.y, P.inv_tol)) {
+                L.flags |= RT_FLAG_PROJECTED;
+                ++L.proj;
+            }
+            L.lambda += best * ah;
+            L.cls = best_cls;
+            L.g = best_g;
+            return;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
+        L.lambda += ah;
+        if (project_multi(T, L.y, P.inv_tol)) {
+            L.flags |= RT_FLAG_PROJECTED;
+            ++L.proj;
+            rhs_multi(T, L.y, L.k1);
+            ++L.rhs;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) L.k1[i] = k7[i];
+        }
+        L.ah = next_step(ah, err, true, P.h_min, P.h_max);
+    }
+}
+
+// Stage the superposition table into LDS (whole workgroup).
+__device__ __forceinline__ void stage_table(MultiDev& sm, const MultiDev* __restrict__ T) {
+    const uint32_t words = sizeof(MultiDev) / 8;
+    const double* src = reinterpret_cast<const double*>(T);
+    double* dst = reinterpret_cast<double*>(&sm);
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+// Store with BL coordinates taken in the frame of source 0 (the central hole).
+__device__ __forceinline__ void store_multi(const OutPtrs& O, uint64_t idx, const Lane& L, const MultiDev& T) {
+    const uint64_t n = O.n;
+    O.cls[idx] = L.cls;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        O.x[c * n + idx] = L.y[c];
+        O.p[c * n + idx] = L.y[4 + c];
+    }
+    double l[3], bl[3];
+    to_local(T.src[0], L.y[1], L.y[2], L.y[3], l);
+    bl_from_ks(l[0], l[1], l[2], T.src[0].a, T.src[0].a2, bl);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) O.bl[c * n + idx] = bl[c];
+    O.lambda[idx] = L.lambda;
+    O.g[idx] = L.g;
+    O.steps[idx] = L.acc;
+    O.rejected[idx] = L.rej;
+}
+
+__global__ __launch_bounds__(256) void k_trace_multi(TraceParams P, OutPtrs O, const MultiDev* __restrict__ Tg) {
+    __shared__ MultiDev sm;
+    stage_table(sm, Tg);
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    Tally tally;
+    bool valid;
+    uint64_t idx = 0;
+    double y0[8];
+    if (P.mode == 0) {
+        const uint32_t tile = static_cast<uint32_t>(t >> 6), l = static_cast<uint32_t>(t & 63);
+        const uint32_t px = (tile % P.tiles_x) * 8 + (l & 7);
+        const uint32_t py = (tile / P.tiles_x) * 8 + (l >> 3);
+        valid = (px < P.width) && (py < P.n_rays / P.width);
+        if (valid) {
+            camera_state_multi(P, sm, px, P.row_first + py, y0);
+            idx = static_cast<uint64_t>(py) * P.width + px;
+        }
+    } else {
+        valid = t < P.n_rays;
+        if (valid) {
+            idx = t;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                y0[c] = O.in_x[c * O.n + t];
+                y0[4 + c] = O.in_p[c * O.n + t];
+            }
+        }
+    }
+    if (valid) {
+        Lane L;
+        if (sm.observer_inside && P.mode == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
+            L.acc = L.rej = L.flags = L.proj = L.f32 = 0;
+            L.rhs = 0;
+            L.lambda = 0.0;
+            L.g = 0.0;
+            L.cls = RT_CLS_CAPTURED;
+        } else {
+            trace_multi(P, sm, y0, L);
+        }
+        lane_finish(L);
+        store_multi(O, idx, L, sm);
+        tally.add(L);
+    }
+    tally.flush(O.counters);
 }
 
 // ------------------------------------------------------------ launchers
+static bool shape_ok(const LaunchShape& L) { return L.block != 0 && L.block <= 256 && (L.block & 63) == 0; }
+
 template <bool Mixed>
 static hipError_t launch_camera_t(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
     const uint64_t tiles_y = (P.n_rays / P.width + 7) / 8;
@@ -384,8 +880,8 @@ static hipError_t launch_camera_t(const TraceParams& P, const OutPtrs& O, const 
 }
 
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
-    if (L.block == 0 || L.block > 256 || (L.block & 63) != 0) return hipErrorInvalidValue;
-    return P.precision == 1 ? launch_camera_t<true>(P, O, L, stream) : launch_camera_t<false>(P, O, L, stream);
+    if (!shape_ok(L) || P.width == 0 || P.n_rays % P.width != 0) return hipErrorInvalidValue;
+    return P.mixed ? launch_camera_t<true>(P, O, L, stream) : launch_camera_t<false>(P, O, L, stream);
 }
 
 template <bool Mixed>
@@ -399,8 +895,69 @@ static hipError_t launch_rays_t(const TraceParams& P, const OutPtrs& O, const La
 }
 
 hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
-    if (L.block == 0 || L.block > 256 || (L.block & 63) != 0) return hipErrorInvalidValue;
-    return P.precision == 1 ? launch_rays_t<true>(P, O, L, stream) : launch_rays_t<false>(P, O, L, stream);
+    if (!shape_ok(L)) return hipErrorInvalidValue;
+    return P.mixed ? launch_rays_t<true>(P, O, L, stream) : launch_rays_t<false>(P, O, L, stream);
+}
+
+hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, hipStream_t stream) {
+    uint64_t threads;
+    if (P.mode == 0) {
+        if (P.width == 0 || P.n_rays % P.width != 0) return hipErrorInvalidValue;
+        threads = static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.width + 7) / 8) * 64;
+    } else {
+        threads = P.n_rays;
+    }
+    const uint32_t block = 256;
+    const uint32_t grid = static_cast<uint32_t>((threads + block - 1) / block);
+    hipLaunchKernelGGL(k_trace_multi, dim3(grid), dim3(block), 0, stream, P, O, T);
+    return hipGetLastError();
+}
+
+template <bool Mixed>
+static hipError_t launch_queue_t(const QueueDev& Q, const LaunchShape& L, uint32_t grid, hipStream_t stream) {
+    if (L.waves >= 2)
+        hipLaunchKernelGGL((k_trace_queue<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, Q);
+    else
+        hipLaunchKernelGGL((k_trace_queue<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, Q);
+    return hipGetLastError();
+}
+
+hipError_t queue_grid(const LaunchShape& L, bool mixed, uint32_t* grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const void* fn = nullptr;
+    if (mixed)
+        fn = (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<true, 2>)
+                            : reinterpret_cast<const void*>(&k_trace_queue<true, 1>);
+    else
+        fn = (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<false, 2>)
+                            : reinterpret_cast<const void*>(&k_trace_queue<false, 1>);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, static_cast<int>(L.block), 0);
+    if (e != hipSuccess) return e;
+    *grid = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu > 0 ? per_cu : 1);
+    return hipSuccess;
+}
+
+hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, uint32_t grid, hipStream_t stream) {
+    if (!shape_ok(L) || Q.n_req == 0) return hipErrorInvalidValue;
+    if (grid == 0) {
+        const hipError_t e = queue_grid(L, mixed, &grid);
+        if (e != hipSuccess) return e;
+    }
+    return mixed ? launch_queue_t<true>(Q, L, grid, stream) : launch_queue_t<false>(Q, L, grid, stream);
+}
+
+// Zero a small counter array with a kernel (safe inside a captured graph).
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
 }
 
 }  // namespace rt

@@ -98,6 +98,10 @@ class rt_request(C.Structure):
                 ("termination", rt_termination)]
 
 
+class rt_launch_shape(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("waves", C.c_uint32), ("refill_min", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class rt_server_config(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32), ("slots", C.c_uint32)]
 
@@ -201,6 +205,12 @@ class Results:
                 "steps": self.steps, "rejected": self.rejected}
 
 
+def make_request(cam: rt_camera, metric: rt_metric, integ: rt_integrator, term: rt_termination) -> rt_request:
+    r = rt_request()
+    r.camera, r.metric, r.integrator, r.termination = cam, metric, integ, term
+    return r
+
+
 def n_rays_of(cam: rt_camera) -> int:
     first = min(cam.row_offset, cam.height)
     last = min(first + cam.row_count, cam.height) if cam.row_count else cam.height
@@ -214,7 +224,7 @@ EXPORTS = [
     "rt_default_integrator", "rt_default_termination", "rt_default_camera", "rt_kerr_metric",
     "rt_abi_version", "rt_version", "rt_last_error", "rt_context_create", "rt_context_destroy",
     "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
-    "rt_trace_batch", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
+    "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
     "rt_server_wait", "rt_server_take", "rt_server_cancel",
 ]
 
@@ -243,6 +253,9 @@ def load(path: Optional[str] = None) -> C.CDLL:
                                              P(rt_termination), P(C.c_size_t)]
     lib.rt_fetch_results.argtypes = [C.c_void_p, P(rt_results)]
     lib.rt_trace_batch.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_results)]
+    lib.rt_get_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
+    lib.rt_set_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
+    lib.rt_autotune.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_launch_shape)]
     lib.rt_default_integrator.argtypes = [P(rt_integrator)]
     lib.rt_default_termination.argtypes = [P(rt_termination)]
     lib.rt_default_camera.argtypes = [P(rt_camera), C.c_double, C.c_double, C.c_double, C.c_double,
@@ -321,12 +334,32 @@ class Context:
                                                 C.byref(integ), C.byref(term), C.byref(res.struct)))
         return res
 
-    def trace_batch(self, reqs: Sequence[rt_request]) -> List[Results]:
+    def trace_batch(self, reqs: Sequence[rt_request], fetch: bool = True) -> Optional[List[Results]]:
+        """All requests in one work-queue kernel (captured hipGraph, replayed
+        while the layout is unchanged). fetch=False keeps results on the device."""
         arr = (rt_request * len(reqs))(*reqs)
+        if not fetch:
+            _check(self.lib, self.lib.rt_trace_batch(self.h, arr, len(reqs), None))
+            return None
         outs = [Results(n_rays_of(r.camera)) for r in reqs]
         oarr = (rt_results * len(reqs))(*[o.struct for o in outs])
         _check(self.lib, self.lib.rt_trace_batch(self.h, arr, len(reqs), oarr))
         return outs
+
+    def launch_shapes(self):
+        plain, queue = rt_launch_shape(), rt_launch_shape()
+        _check(self.lib, self.lib.rt_get_launch_shape(self.h, C.byref(plain), C.byref(queue)))
+        return plain, queue
+
+    def set_launch_shapes(self, plain: Optional[rt_launch_shape] = None, queue: Optional[rt_launch_shape] = None):
+        _check(self.lib, self.lib.rt_set_launch_shape(self.h, C.byref(plain) if plain else None,
+                                                      C.byref(queue) if queue else None))
+
+    def autotune(self, sample: Sequence[rt_request]) -> rt_launch_shape:
+        arr = (rt_request * len(sample))(*sample)
+        best = rt_launch_shape()
+        _check(self.lib, self.lib.rt_autotune(self.h, arr, len(sample), C.byref(best)))
+        return best
 
 
 class Server:

@@ -1,0 +1,57 @@
+"""Serving loop: continuous batching on the device, admission by HBM footprint.
+
+Every request's result must be the same alone (single-request kernel) and in
+the mix of concurrently served requests (class identical, coordinates within
+1e-12 relative); a request too large for the budget is refused with
+RT_ERR_BUSY; a cancelled pending request reports RT_ERR_CANCELLED."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+import rtsim
+
+pytestmark = pytest.mark.gpu
+
+
+def req(side, a, inc, r_obs=1000.0, rtol=1e-10):
+    cam = rtsim.default_camera(r_obs, math.radians(inc), 0.0, math.radians(20.0), side, side)
+    return rtsim.make_request(cam, rtsim.kerr_metric(1.0, a), rtsim.default_integrator(rtol=rtol),
+                              rtsim.default_termination(escape_radius=max(1000.0, r_obs)))
+
+
+def test_alone_equals_mix(ctx):
+    reqs = [req(48, 0.9, 60), req(33, 0.0, 20), req(64, 0.998, 85), req(40, 0.5, 45, 300.0, 1e-8),
+            req(16, 0.3, 70)]
+    alone = [ctx.trace_camera(r.camera, r.metric, r.integrator, r.termination) for r in reqs]
+    with rtsim.Server(0, hbm_budget_bytes=64 << 20, max_inflight=8) as srv:
+        ids = []
+        for r in reqs:
+            ids.append(srv.submit(r))
+            time.sleep(0.002)  # arrivals while earlier requests run
+        for i, rid in enumerate(ids):
+            srv.wait(rid, 60000)
+            got = srv.take(rid)
+            assert np.array_equal(got.cls, alone[i].cls), f"request {i}: class differs"
+            for name in ("x", "p", "bl", "lam", "g"):
+                u, v = getattr(got, name), getattr(alone[i], name)
+                d = (np.abs(u - v) / np.maximum(np.abs(v), 1.0)).max(initial=0)
+                assert d < 1e-12, f"request {i}: {name} differs by {d:.3e}"
+
+
+def test_admission_and_cancel():
+    with rtsim.Server(0, hbm_budget_bytes=2 << 20, max_inflight=1) as srv:
+        with pytest.raises(rtsim.RtError) as e:
+            srv.submit(req(512, 0.5, 60))  # ~30 MB of results: never fits 2 MiB
+        assert e.value.status == 7
+        first = srv.submit(req(48, 0.9, 60))
+        second = srv.submit(req(40, 0.9, 60))
+        srv.cancel(second)
+        srv.wait(first, 60000)
+        srv.take(first)
+        with pytest.raises(rtsim.RtError) as e2:
+            srv.wait(second, 60000)
+        assert e2.value.status == 5
+        with pytest.raises(rtsim.RtError):
+            srv.poll(123456)
