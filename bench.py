@@ -73,6 +73,113 @@ def cpu_baseline(side: int, stride: int, threads: int):
                       f"oracle/{os.path.basename(oracle_lib.ORACLE_PATH)}"}
 
 
+def serving_requests(n: int, seed: int):
+    """Trace requests of the serving workload: lognormal ray counts (median
+    256^2, sigma 1, at most 2048^2, square frames), 70% single Kerr with
+    a ~ U[0, 0.998], 30% superpositions of 2..16 holes (masses U[0.1, 1],
+    positions U[-25, 25]^3 M, spins U[0, 0.95] on random axes); observer at
+    r = 1000 M, inclination U[5, 89] deg, FOV 20 deg, rtol 1e-10."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        rays = min(2048 * 2048, int(round(65536 * math.exp(rng.normal()))))
+        side = max(8, int(round(math.sqrt(rays))))
+        cam = rtsim.default_camera(R_OBS, math.radians(rng.uniform(5, 89)), rng.uniform(-math.pi, math.pi),
+                                   math.radians(FOV_DEG), side, side)
+        if rng.random() < 0.3:
+            k = int(rng.integers(2, 17))
+            src = [{"mass": 1.0, "spin": float(rng.uniform(0, 0.95))}]
+            for _ in range(k - 1):
+                src.append({"mass": float(rng.uniform(0.1, 1.0)), "spin": float(rng.uniform(0, 0.95)),
+                            "position": tuple(float(v) for v in rng.uniform(-25, 25, size=3)),
+                            "axis": tuple(float(v) for v in rng.normal(size=3))})
+            met = rtsim.multi_metric(src)
+        else:
+            met = rtsim.kerr_metric(1.0, float(rng.uniform(0, 0.998)))
+        out.append(rtsim.make_request(cam, met, rtsim.default_integrator(), rtsim.default_termination()))
+    return out
+
+
+def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
+    """Poisson arrivals at a fixed QPS into the serving loop; latency is
+    submit -> results on the host."""
+    reqs = serving_requests(n, seed)
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
+    lat = []
+    rays = 0
+    with rtsim.Server(0, hbm_budget_bytes=16 << 30, max_inflight=256) as srv:
+        pending = {}
+        t0 = time.perf_counter()
+        i = 0
+        while i < n or pending:
+            now = time.perf_counter() - t0
+            while i < n and arrivals[i] <= now:
+                pending[srv.submit(reqs[i])] = (time.perf_counter(), rtsim.n_rays_of(reqs[i].camera))
+                i += 1
+            for rid in [r for r in pending if srv.poll(r)]:
+                ts, nr = pending.pop(rid)
+                srv.take(rid)
+                lat.append(time.perf_counter() - ts)
+                rays += nr
+            time.sleep(0.0002)
+        wall = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+    return {"qps": qps, "requests": n, "p50_ms": float(np.percentile(lat_ms, 50)),
+            "p99_ms": float(np.percentile(lat_ms, 99)), "mean_ms": float(lat_ms.mean()),
+            "rays_per_s": rays / wall, "multi_source_fraction": 0.3,
+            "workload": "Poisson arrivals, lognormal rays (median 256^2, sigma 1, max 2048^2), "
+                        "70% Kerr / 30% 2-16 hole superpositions"}
+
+
+def uneven_requests(n: int = 512, seed: int = 0xC0FFEE1234):
+    """Batched workload: sides U[64, 1024], a U[0, 0.998], inclination
+    U[5, 89] deg, r_obs log-uniform in [50, 5000] M, rtol in {1e-8, 1e-10}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        side = int(rng.integers(64, 1025))
+        r_obs = float(math.exp(rng.uniform(math.log(50.0), math.log(5000.0))))
+        cam = rtsim.default_camera(r_obs, math.radians(rng.uniform(5, 89)), 0.0, math.radians(FOV_DEG), side, side)
+        rtol = 1e-8 if rng.random() < 0.5 else 1e-10
+        out.append(rtsim.make_request(cam, rtsim.kerr_metric(1.0, float(rng.uniform(0, 0.998))),
+                                      rtsim.default_integrator(rtol=rtol),
+                                      rtsim.default_termination(escape_radius=max(1000.0, r_obs))))
+    return out
+
+
+def precision_report(side: int, switch_radius: float):
+    """Mixed precision vs FP64 on the GPU: Kerr a = 0.998, inclination 85 deg,
+    side x side camera (4096 by default). Per-output error of the mixed run
+    against the FP64 run, and class agreement."""
+    cam = rtsim.default_camera(R_OBS, math.radians(85.0), 0.0, math.radians(FOV_DEG), side, side)
+    met = rtsim.kerr_metric(1.0, 0.998)
+    tm = rtsim.default_termination()
+    ctx = rtsim.Context(0)
+    rep = {"workload": f"kerr_a0.998_i85_{side}sq", "switch_radius": switch_radius}
+    ref = ctx.trace_camera(cam, met, rtsim.default_integrator(), tm)
+    rep["fp64_kernel_ms"] = ctx.stats().kernel_ms
+    mx = ctx.trace_camera(cam, met, rtsim.default_integrator(precision=rtsim.RT_PRECISION_MIXED,
+                                                             switch_radius=switch_radius), tm)
+    st = ctx.stats()
+    rep["mixed_kernel_ms"] = st.kernel_ms
+    rep["fp32_step_fraction"] = st.fp32_steps / max(1, st.accepted_steps + st.rejected_steps)
+    same = mx.klass == ref.klass
+    rep["class_agreement"] = float(same.mean())
+    rep["class_mismatches"] = int((~same).sum())
+    ok = same & (ref.klass != rtsim.RT_CLS_CAPTURED)
+    den = lambda v: np.maximum(np.abs(v), 1.0)  # noqa: E731
+    errs = {"x": np.max(np.abs(mx.x[1:, ok] - ref.x[1:, ok]) / den(ref.x[1:, ok]), axis=0),
+            "r": np.abs(mx.bl[0, ok] - ref.bl[0, ok]) / den(ref.bl[0, ok]),
+            "theta": np.abs(mx.bl[1, ok] - ref.bl[1, ok]) / den(ref.bl[1, ok]),
+            "phi": np.abs(np.angle(np.exp(1j * (mx.bl[2, ok] - ref.bl[2, ok])))) / np.pi,
+            "g": np.abs(mx.g[ok] - ref.g[ok]) / den(ref.g[ok])}
+    rep["rel_err"] = {k: {"p50": float(np.median(v)), "p99": float(np.percentile(v, 99)), "max": float(v.max()),
+                          "frac_le_1e-9": float((v <= 1e-9).mean())} for k, v in errs.items()}
+    ctx.close()
+    return rep
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,13 +193,26 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--qps", type=float, default=50.0)
+    ap.add_argument("--serve-requests", type=int, default=150)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--workload", choices=["frames4", "uneven512"], default="frames4",
+                    help="frames4: the four 2048^2 frames; uneven512: 512 seeded camera requests in one batch")
+    ap.add_argument("--precision-report", type=int, default=0, metavar="SIDE",
+                    help="print a mixed-vs-FP64 error report at this camera side (e.g. 4096) and exit")
+    ap.add_argument("--switch-radius", type=float, default=100.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this tracer runs on one GPU; --gpus {args.gpus} is not supported "
               "(rays shard with zero exchange, see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.precision_report:
+        print(json.dumps(precision_report(args.precision_report, args.switch_radius)), flush=True)
+        return 0
     prec = rtsim.RT_PRECISION_MIXED if args.precision == "mixed" else rtsim.RT_PRECISION_FP64
     reqs = frame_requests(args.side, prec)
+    if args.workload == "uneven512":
+        reqs = [(r.camera, r.metric, r.integrator, r.termination) for r in uneven_requests()]
     ctx = rtsim.Context(0)
     print(f"bench: {len(reqs)} frames of {args.side}^2, precision {args.precision}, "
           f"warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
@@ -148,7 +268,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64" if prec == rtsim.RT_PRECISION_FP64 else "f64+f32",
         "data": "synthetic (camera rays generated on device)",
-        "config": {"workload": f"kerr_{args.side}sq_4spins_i60_robs1000_fov20", "side": args.side,
+        "config": {"workload": (f"kerr_{args.side}sq_4spins_i60_robs1000_fov20" if args.workload == "frames4"
+                                else "uneven512_seed_c0ffee1234"), "side": args.side,
                    "spins": list(SPINS), "precision": args.precision, "rays_per_step": rays // args.steps,
                    "mean_accepted_steps": acc / rays, "mean_rejected_steps": rej / rays,
                    "rhs_evals_per_ray": rhs / rays, "path": args.path, "kernel_ms_per_launch": kernel_ms / launches,
@@ -157,6 +278,9 @@ def main() -> int:
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
     }
     ctx.close()
+    if not args.no_serve:
+        print(f"bench: serving {args.serve_requests} requests at {args.qps} QPS", file=sys.stderr, flush=True)
+        out["latency"] = serve_latency(args.qps, args.serve_requests)
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         print(f"bench: cpu baseline on {threads} threads", file=sys.stderr, flush=True)

@@ -216,8 +216,9 @@ rt_status rt_fetch_results(rt_context* ctx, rt_results* out);
 typedef struct rt_launch_shape {
     uint32_t block;       /* threads per workgroup: 64, 128, 192 or 256          */
     uint32_t waves;       /* occupancy the kernel is built for: 1 or 2 waves/SIMD */
-    uint32_t refill_min;  /* work queue: idle lanes that trigger a refill (1..64) */
-    uint32_t reserved;
+    uint32_t refill_min;  /* lane-refill queue: idle lanes that trigger a refill (1..64) */
+    uint32_t lane_refill; /* work queue: 0 = a wave takes whole 8x8 tiles,
+                             1 = idle lanes are refilled ray by ray           */
 } rt_launch_shape;
 
 /* Shapes used by the per-frame kernels (plain) and the work-queue kernel. */

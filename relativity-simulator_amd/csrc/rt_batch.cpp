@@ -37,7 +37,7 @@ size_t al256(size_t v) { return (v + 255) & ~size_t(255); }
 struct BatchPlan {
     std::vector<uint32_t> q_req;         // request index of each queue entry
     std::vector<TraceParams> q_params;
-    std::vector<uint64_t> q_offset;      // n_q + 1
+    std::vector<uint64_t> q_offset;      // n_q + 1, in rays (lane refill) or tiles
     std::vector<uint32_t> m_req;         // request index of each superposition entry
     std::vector<TraceParams> m_params;
     std::vector<MultiDev> m_tables;
@@ -46,8 +46,9 @@ struct BatchPlan {
     size_t result_bytes = 0, table_bytes = 0;
 };
 
-// Validate and lay out a batch (no device work).
-rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan) {
+// Validate and lay out a batch (no device work). tiles: queue entries are
+// 8x8 tiles (k_trace_tiles) rather than rays (k_trace_queue).
+rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool tiles) {
     plan = BatchPlan{};
     plan.n_rays.resize(n);
     plan.q_offset.push_back(0);
@@ -70,7 +71,8 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan) {
             plan.q_req.push_back(static_cast<uint32_t>(i));
             plan.q_params.push_back(P);
             const uint64_t tiles_y = (P.n_rays / P.width + 7) / 8;
-            plan.q_offset.push_back(plan.q_offset.back() + static_cast<uint64_t>(P.tiles_x) * tiles_y * 64);
+            const uint64_t n_tiles = static_cast<uint64_t>(P.tiles_x) * tiles_y;
+            plan.q_offset.push_back(plan.q_offset.back() + (tiles ? n_tiles : n_tiles * 64));
             plan.mixed = plan.mixed || P.mixed;
         }
         plan.n_rays[i] = P.n_rays;
@@ -218,7 +220,7 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
     const auto t0 = std::chrono::steady_clock::now();
     RT_HIP(hipSetDevice(ctx->device));
     BatchPlan plan;
-    rt_status st = plan_batch(reqs, n_req, plan);
+    rt_status st = plan_batch(reqs, n_req, plan, ctx->qshape.lane_refill == 0);
     if (st != RT_OK) return st;
     st = ctx->arena.ensure(std::max<size_t>(plan.result_bytes, 256));
     if (st != RT_OK) return st;
@@ -400,6 +402,11 @@ void rt_server::launch_locked(std::unique_lock<std::mutex>& lk) {
             cv_done.notify_all();
             continue;
         }
+        // A superposition request runs as its own batch (on its own stream),
+        // so the long multi-source kernels do not hold back the single-Kerr
+        // requests admitted with them.
+        const bool multi = r.req.metric.n_sources >= 2;
+        if (!take.empty() && (multi || batch.front().metric.n_sources >= 2)) break;
         size_t off_probe = 0;
         const size_t want = bytes + r.bytes;
         if (!alloc.alloc(want, &off_probe)) break;
@@ -408,6 +415,7 @@ void rt_server::launch_locked(std::unique_lock<std::mutex>& lk) {
         batch.push_back(r.req);
         bytes = want;
         pending.pop_front();
+        if (multi) break;
     }
     if (take.empty()) return;
     Inflight* b = new Inflight();
@@ -421,7 +429,7 @@ void rt_server::launch_locked(std::unique_lock<std::mutex>& lk) {
     lk.unlock();
     // plan + place + upload + launch outside the lock
     BatchPlan plan;
-    rt_status st = plan_batch(batch.data(), batch.size(), plan);
+    rt_status st = plan_batch(batch.data(), batch.size(), plan, qshape.lane_refill == 0);
     hipError_t he = hipSuccess;
     if (st == RT_OK) {
         char* base = static_cast<char*>(pool) + b->off;
@@ -526,14 +534,14 @@ extern "C" {
 
 rt_status rt_get_launch_shape(const rt_context* ctx, rt_launch_shape* plain, rt_launch_shape* queue) {
     if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
-    if (plain) *plain = rt_launch_shape{ctx->shape.block, ctx->shape.waves, ctx->shape.refill_min, 0};
-    if (queue) *queue = rt_launch_shape{ctx->qshape.block, ctx->qshape.waves, ctx->qshape.refill_min, 0};
+    if (plain) *plain = rt_launch_shape{ctx->shape.block, ctx->shape.waves, ctx->shape.refill_min, ctx->shape.lane_refill};
+    if (queue) *queue = rt_launch_shape{ctx->qshape.block, ctx->qshape.waves, ctx->qshape.refill_min, ctx->qshape.lane_refill};
     return RT_OK;
 }
 
 static bool shape_valid(const rt_launch_shape* s) {
     return s->block >= 64 && s->block <= 256 && (s->block % 64) == 0 && (s->waves == 1 || s->waves == 2) &&
-           s->refill_min >= 1 && s->refill_min <= 64;
+           s->refill_min >= 1 && s->refill_min <= 64 && s->lane_refill <= 1;
 }
 
 rt_status rt_set_launch_shape(rt_context* ctx, const rt_launch_shape* plain, const rt_launch_shape* queue) {
@@ -541,8 +549,8 @@ rt_status rt_set_launch_shape(rt_context* ctx, const rt_launch_shape* plain, con
     if ((plain && !shape_valid(plain)) || (queue && !shape_valid(queue)))
         return fail(RT_ERR_INVALID_ARG, "launch shape: block in {64,128,192,256}, waves in {1,2}, refill_min 1..64");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    if (plain) ctx->shape = LaunchShape{plain->block, plain->waves, plain->refill_min};
-    if (queue) ctx->qshape = LaunchShape{queue->block, queue->waves, queue->refill_min};
+    if (plain) ctx->shape = LaunchShape{plain->block, plain->waves, plain->refill_min, plain->lane_refill};
+    if (queue) ctx->qshape = LaunchShape{queue->block, queue->waves, queue->refill_min, queue->lane_refill};
     return RT_OK;
 }
 
@@ -557,29 +565,32 @@ rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample
     for (size_t i = 0; i < n_sample; ++i)
         if (sample[i].metric.n_sources >= 2) return fail(RT_ERR_INVALID_ARG, "autotune takes single-Kerr requests");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    const uint32_t blocks[] = {64, 128, 256};
-    const uint32_t waves[] = {1, 2};
-    const uint32_t refills[] = {8, 16, 32};
+    // candidates: tile queue at each block size and occupancy, lane-refill
+    // queue at each block size with three refill thresholds
+    std::vector<LaunchShape> cands;
+    for (uint32_t b : {64u, 128u, 256u})
+        for (uint32_t w : {1u, 2u}) {
+            cands.push_back(LaunchShape{b, w, 16, 0});
+            for (uint32_t r : {8u, 16u, 32u}) cands.push_back(LaunchShape{b, w, r, 1});
+        }
     LaunchShape keep = ctx->qshape, win = ctx->qshape;
     double best_ms = 1e300;
-    for (uint32_t b : blocks)
-        for (uint32_t w : waves)
-            for (uint32_t r : refills) {
-                ctx->qshape = LaunchShape{b, w, r};
-                rt_status st = run_batch(ctx, sample, n_sample, nullptr);  // capture + run
-                if (st == RT_OK) st = run_batch(ctx, sample, n_sample, nullptr);  // replay, timed
-                if (st != RT_OK) {
-                    ctx->qshape = keep;
-                    return st;
-                }
-                if (ctx->stats.kernel_ms < best_ms) {
-                    best_ms = ctx->stats.kernel_ms;
-                    win = ctx->qshape;
-                }
-            }
+    for (const LaunchShape& c : cands) {
+        ctx->qshape = c;
+        rt_status st = run_batch(ctx, sample, n_sample, nullptr);  // capture + run
+        if (st == RT_OK) st = run_batch(ctx, sample, n_sample, nullptr);  // replay, timed
+        if (st != RT_OK) {
+            ctx->qshape = keep;
+            return st;
+        }
+        if (ctx->stats.kernel_ms < best_ms) {
+            best_ms = ctx->stats.kernel_ms;
+            win = ctx->qshape;
+        }
+    }
     ctx->qshape = win;
     ctx->tuned = true;
-    if (best) *best = rt_launch_shape{win.block, win.waves, win.refill_min, 0};
+    if (best) *best = rt_launch_shape{win.block, win.waves, win.refill_min, win.lane_refill};
     return RT_OK;
 }
 

@@ -50,16 +50,14 @@ __device__ __forceinline__ double rcp64(double x) {
     return fma(r, e, r);
 }
 
-// sqrt(x) and 1/sqrt(x) of a positive normal x: v_rsq_f64 and two Goldschmidt
-// iterations plus a final residual correction of the root.
+// sqrt(x) and 1/sqrt(x) of a positive normal x: v_rsq_f64 (about 2^-23
+// relative), one Goldschmidt iteration (2^-46) and a residual correction of
+// the root; within 1 ulp.
 __device__ __forceinline__ double sqrt_rsqrt64(double x, double& rs) {
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y;
     double h = 0.5 * y;
-    double r = fma(-g, h, 0.5);
-    g = fma(g, r, g);
-    h = fma(h, r, h);
-    r = fma(-g, h, 0.5);
+    const double r = fma(-g, h, 0.5);
     g = fma(g, r, g);
     h = fma(h, r, h);
     const double d = fma(-g, g, x);
@@ -255,14 +253,28 @@ __device__ __forceinline__ double disk_redshift(double mass, double spin, double
     return clamp_g ? fmin(fmax(g, 0.05), 4.0) : g;
 }
 
+// e^(-1/5) and e^(-1/4) of e in [1e-12, ...): FP32 seed (v_log_f32 /
+// v_exp_f32, about 3e-7 relative) and one Newton step on y^n e = 1 in FP64
+// (about 6e-13 relative), in place of a full FP64 pow.
+__device__ __forceinline__ double inv_root5(double e) {
+    const double y = static_cast<double>(__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(static_cast<float>(e))));
+    const double y2 = y * y;
+    return y * fma(1.0 - y2 * y2 * y * e, 0.2, 1.0);
+}
+__device__ __forceinline__ double inv_root4(double e) {
+    const double y = static_cast<double>(__builtin_amdgcn_exp2f(-0.25f * __builtin_amdgcn_logf(static_cast<float>(e))));
+    const double y2 = y * y;
+    return y * fma(1.0 - y2 * y2 * e, 0.25, 1.0);
+}
+
 // Standard integral controller with safety 0.9; returns the new |h|.
 __device__ __forceinline__ double next_step(double ah, double err, bool accepted, double h_min, double h_max) {
     if (accepted) {
-        const double factor = (err == 0.0) ? 5.0 : pow(1.0 / fmax(err, 1e-12), 0.2);
+        const double factor = (err == 0.0) ? 5.0 : inv_root5(fmax(err, 1e-12));
         const double s = fmin(fmax(0.9 * factor, 0.2), 5.0);
         return fmin(fmax(ah * s, h_min), h_max);
     }
-    const double factor = pow(1.0 / fmax(err, 1e-12), 0.25);
+    const double factor = inv_root4(fmax(err, 1e-12));
     return ah * fmin(fmax(0.9 * factor, 0.1), 0.9);
 }
 

@@ -572,9 +572,6 @@ rt_status rt_context_create(int device, rt_context** out) {
     rt_context* ctx = new (std::nothrow) rt_context();
     if (!ctx) return fail(RT_ERR_OOM, "context allocation");
     ctx->device = device;
-    // Launch-shape override for experiments: RTSIM_BLOCK (64..256), RTSIM_WAVES (1|2).
-    if (const char* b = std::getenv("RTSIM_BLOCK")) ctx->shape.block = static_cast<uint32_t>(std::atoi(b));
-    if (const char* w = std::getenv("RTSIM_WAVES")) ctx->shape.waves = static_cast<uint32_t>(std::atoi(w));
     ctx->qshape = ctx->shape;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);

@@ -12,9 +12,10 @@ namespace rt {
 // and, for the work-queue kernel, how many lanes of a wave must be idle
 // before it refills them.
 struct LaunchShape {
-    uint32_t block = 256;
-    uint32_t waves = 1;
+    uint32_t block = 64;
+    uint32_t waves = 2;
     uint32_t refill_min = 16;
+    uint32_t lane_refill = 0;  // work queue: 0 whole 8x8 tiles per wave, 1 per-lane refill
 };
 
 // Camera frame (single Kerr): one lane per pixel, lanes of a wave on an 8x8 tile.
@@ -23,7 +24,8 @@ hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const Lau
 hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
 // Superposition (camera or explicit rays per P.mode), table T in device memory.
 hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, hipStream_t stream);
-// Persistent work-queue kernel over many single-Kerr requests.
+// Persistent work-queue kernel over many single-Kerr requests (tiles or rays
+// per L.lane_refill; Q.req_offset counts tiles or rays accordingly).
 // grid 0: computed from the occupancy (not allowed inside a stream capture).
 hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, uint32_t grid, hipStream_t stream);
 // Grid that fills the device at the kernel's occupancy.

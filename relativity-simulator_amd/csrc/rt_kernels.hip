@@ -7,9 +7,11 @@
 //   k_trace_camera_kerr  one camera frame, one lane per pixel, the lanes of a
 //                        wave on an 8x8 pixel tile (similar step counts)
 //   k_trace_rays_kerr    explicit initial states
-//   k_trace_queue        persistent kernel over a device work queue of many
-//                        requests: a wave refills its idle lanes from the
-//                        queue with one atomic (ballot + prefix count)
+//   k_trace_tiles        persistent kernel over a device work queue of 8x8
+//                        tiles of many requests (one atomic per tile)
+//   k_trace_queue        persistent kernel over a device work queue of rays:
+//                        a wave refills its idle lanes with one atomic
+//                        (ballot + prefix count)
 //   k_trace_multi        superposition of up to 16 sources, table in LDS
 #include <hip/hip_runtime.h>
 
@@ -133,16 +135,19 @@ __device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const dou
         dp45_stages<true, float>(y, k1, h, M, a, a2, y5, ep, k7, dn);
     else
         dp45_stages<true, double>(y, k1, h, M, a, a2, y5, ep, k7, dn);
-    Dense q[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        q[i].D = y5[i] - y[i];
-        q[i].B = h * k1[i] - q[i].D;
-        q[i].C = q[i].D - h * k7[i] - q[i].B;
-        q[i].E = h * (dn[i] + dp::d7 * k7[i]);
-    }
+    // dense-output coefficients of the position only; other components are
+    // evaluated one at a time at the end (keeps the event path's registers low)
+    auto coef = [&](int i) {
+        Dense q;
+        q.D = y5[i] - y[i];
+        q.B = h * k1[i] - q.D;
+        q.C = q.D - h * k7[i] - q.B;
+        q.E = h * (dn[i] + dp::d7 * k7[i]);
+        return q;
+    };
+    const Dense qx = coef(1), qy = coef(2), qz = coef(3);
     auto rad_at = [&](double ss) {
-        return ks_radius(dense_at(q[1], y[1], ss), dense_at(q[2], y[2], ss), dense_at(q[3], y[3], ss), a2);
+        return ks_radius(dense_at(qx, y[1], ss), dense_at(qy, y[2], ss), dense_at(qz, y[3], ss), a2);
     };
     double best = 2.0, best_g = 0.0;
     uint32_t best_cls = 0;
@@ -151,7 +156,7 @@ __device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const dou
         best_cls = RT_CLS_CAPTURED;
     }
     if (dk) {
-        const double sd = illinois_root([&](double ss) { return dense_at(q[3], y[3], ss); }, y[3], y5[3]);
+        const double sd = illinois_root([&](double ss) { return dense_at(qz, y[3], ss); }, y[3], y5[3]);
         const double rc = rad_at(sd);
         if (rc >= P.disk_in && rc <= P.disk_out && rc > P.rh102 && sd < best) {
             best = sd;
@@ -174,7 +179,7 @@ __device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const dou
         for (int i = 0; i < 8; ++i) yend[i] = y5[i];
     } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) yend[i] = dense_at(q[i], y[i], best);
+        for (int i = 0; i < 8; ++i) yend[i] = dense_at(coef(i), y[i], best);
     }
     *s_out = best;
     *g_out = best_g;
@@ -479,6 +484,77 @@ __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
         }
     }
     tally.flush(Q.counters);
+}
+
+// Tile-granular work queue: a wave takes one 8x8 tile (64 rays of one
+// request) per atomicAdd and traces it to the end before taking the next. The
+// request index is wave-uniform (readfirstlane), so its parameters load into
+// scalar registers as in the single-request kernel; the persistent loop
+// refills a wave's slot the moment its tile is done. Q.req_offset counts tiles.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// The request tables come in as separate read-only, non-aliased arguments so
+// that their wave-uniform loads stay scalar loads the compiler may repeat
+// instead of holding the parameters in vector registers.
+template <bool Mixed, int Waves>
+__global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const TraceParams* __restrict__ reqs,
+                                                            const OutPtrs* __restrict__ outs,
+                                                            const uint64_t* __restrict__ req_offset) {
+    const uint32_t lane = threadIdx.x & 63;
+    // statistics accumulate per tile as wave sums, in scalar registers
+    unsigned long long s_acc = 0, s_rej = 0, s_rhs = 0, s_f32 = 0, s_proj = 0;
+    for (;;) {
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(Q.next, 1ull);
+        t = uniform_u64(__shfl(t, 0, 64));
+        if (t >= Q.total) break;
+        // batches hold camera requests only (plan_batch); tile t of request req
+        uint32_t lo = 0, hi = Q.n_req;  // req_offset[lo] <= t < req_offset[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (req_offset[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t req = lo;
+        const uint64_t tile = t - req_offset[req];
+        const TraceParams& P = reqs[req];
+        const uint32_t px = static_cast<uint32_t>(tile % P.tiles_x) * 8 + (lane & 7);
+        const uint32_t py = static_cast<uint32_t>(tile / P.tiles_x) * 8 + (lane >> 3);
+        uint32_t c_acc = 0, c_rej = 0, c_rhs = 0, c_f32 = 0, c_proj = 0;
+        if (px < P.width && py < P.n_rays / P.width) {
+            double y0[8], xi = 0.0;
+            camera_state_kerr(P, px, P.row_first + py, y0, &xi);
+            Lane L;
+            lane_begin(P, y0, xi, L);
+            while (!lane_step<Mixed>(P, L)) {
+            }
+            lane_finish(L);
+            store_lane(outs[req], static_cast<uint64_t>(py) * P.width + px, L, P.a, P.a2);
+            c_acc = L.acc;
+            c_rej = L.rej;
+            c_rhs = L.rhs;
+            c_f32 = L.f32;
+            c_proj = L.proj;
+        }
+        s_acc += uniform_u64(wave_sum(c_acc));
+        s_rej += uniform_u64(wave_sum(c_rej));
+        s_rhs += uniform_u64(wave_sum(c_rhs));
+        s_f32 += uniform_u64(wave_sum(c_f32));
+        s_proj += uniform_u64(wave_sum(c_proj));
+    }
+    if (lane == 0) {
+        atomicAdd(&Q.counters[0], s_acc);
+        atomicAdd(&Q.counters[1], s_rej);
+        atomicAdd(&Q.counters[2], s_rhs);
+        atomicAdd(&Q.counters[3], s_f32);
+        atomicAdd(&Q.counters[4], s_proj);
+    }
 }
 
 // ------------------------------------------------------- superposition
@@ -914,11 +990,29 @@ hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const Mult
 }
 
 template <bool Mixed>
+static const void* queue_fn(const LaunchShape& L) {
+    if (L.lane_refill)
+        return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<Mixed, 2>)
+                              : reinterpret_cast<const void*>(&k_trace_queue<Mixed, 1>);
+    return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_tiles<Mixed, 2>)
+                          : reinterpret_cast<const void*>(&k_trace_tiles<Mixed, 1>);
+}
+
+template <bool Mixed>
 static hipError_t launch_queue_t(const QueueDev& Q, const LaunchShape& L, uint32_t grid, hipStream_t stream) {
-    if (L.waves >= 2)
-        hipLaunchKernelGGL((k_trace_queue<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, Q);
-    else
-        hipLaunchKernelGGL((k_trace_queue<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, Q);
+    if (L.lane_refill) {
+        if (L.waves >= 2)
+            hipLaunchKernelGGL((k_trace_queue<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, Q);
+        else
+            hipLaunchKernelGGL((k_trace_queue<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, Q);
+    } else {
+        if (L.waves >= 2)
+            hipLaunchKernelGGL((k_trace_tiles<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, Q, Q.reqs, Q.outs,
+                               Q.req_offset);
+        else
+            hipLaunchKernelGGL((k_trace_tiles<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, Q, Q.reqs, Q.outs,
+                               Q.req_offset);
+    }
     return hipGetLastError();
 }
 
@@ -927,13 +1021,7 @@ hipError_t queue_grid(const LaunchShape& L, bool mixed, uint32_t* grid) {
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    const void* fn = nullptr;
-    if (mixed)
-        fn = (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<true, 2>)
-                            : reinterpret_cast<const void*>(&k_trace_queue<true, 1>);
-    else
-        fn = (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<false, 2>)
-                            : reinterpret_cast<const void*>(&k_trace_queue<false, 1>);
+    const void* fn = mixed ? queue_fn<true>(L) : queue_fn<false>(L);
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, static_cast<int>(L.block), 0);
     if (e != hipSuccess) return e;
     *grid = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu > 0 ? per_cu : 1);

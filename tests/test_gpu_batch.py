@@ -1,5 +1,6 @@
-"""Batched tracing: one persistent work-queue kernel over uneven requests,
-captured into a hipGraph and replayed.
+"""Batched tracing: one persistent work-queue kernel over uneven requests
+(whole 8x8 tiles per wave, or per-lane refill), captured into a hipGraph and
+replayed.
 
 Each request's results must equal the single-request kernel's results for the
 same request (class identical, coordinates within 1e-12 relative: the same
@@ -39,7 +40,17 @@ def same(a, b, label):
     assert np.array_equal(a.steps, b.steps) and np.array_equal(a.rejected, b.rejected), f"{label}: step counts"
 
 
-def test_batch_matches_single_and_graph_replays(ctx):
+@pytest.mark.parametrize("lane_refill", [0, 1], ids=["tile_queue", "lane_refill_queue"])
+def test_batch_matches_single_and_graph_replays(ctx, lane_refill):
+    plain, queue = ctx.launch_shapes()
+    ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(128, 2, 16, lane_refill))
+    try:
+        check_batch_replays(ctx)
+    finally:
+        ctx.set_launch_shapes(queue=queue)
+
+
+def check_batch_replays(ctx):
     reqs = uneven_requests()
     eager = [ctx.trace_camera(r.camera, r.metric, r.integrator, r.termination) for r in reqs]
     for rep in range(3):
@@ -57,8 +68,9 @@ def test_batch_matches_single_and_graph_replays(ctx):
 def test_autotune_then_batch(ctx):
     reqs = uneven_requests()[:3]
     best = ctx.autotune(reqs)
-    print(f"autotuned block={best.block} waves={best.waves} refill={best.refill_min}", flush=True)
-    assert best.block in (64, 128, 256) and best.waves in (1, 2)
+    print(f"autotuned block={best.block} waves={best.waves} refill={best.refill_min} lane_refill={best.lane_refill}",
+          flush=True)
+    assert best.block in (64, 128, 256) and best.waves in (1, 2) and best.lane_refill in (0, 1)
     eager = [ctx.trace_camera(r.camera, r.metric, r.integrator, r.termination) for r in reqs]
     for o, e in zip(ctx.trace_batch(reqs), eager):
         same(o, e, "after autotune")
@@ -66,10 +78,12 @@ def test_autotune_then_batch(ctx):
 
 def test_launch_shape_roundtrip(ctx):
     plain, queue = ctx.launch_shapes()
-    s = rtsim.rt_launch_shape(128, 2, 8, 0)
+    s = rtsim.rt_launch_shape(128, 2, 8, 1)
     ctx.set_launch_shapes(queue=s)
     _, q2 = ctx.launch_shapes()
-    assert (q2.block, q2.waves, q2.refill_min) == (128, 2, 8)
+    assert (q2.block, q2.waves, q2.refill_min, q2.lane_refill) == (128, 2, 8, 1)
     with pytest.raises(rtsim.RtError):
         ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(100, 3, 0, 0))
+    with pytest.raises(rtsim.RtError):
+        ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(64, 1, 16, 2))
     ctx.set_launch_shapes(plain=plain, queue=queue)

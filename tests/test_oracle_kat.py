@@ -254,3 +254,41 @@ def test_schwarzschild_shadow_edge():
             assert res.klass[px] == 1, (px, ang, alpha)
         elif ang > 1.03 * alpha:
             assert res.klass[px] == 2, (px, ang, alpha)
+
+
+# ------------------------------------------------- Carter radial classifier
+def carter_captured(xi, eta, a, M=1.0):
+    """Integration-free class of a ray from afar with constants (xi, eta)
+    (CarterPhaseClassifier, optics/carter_ray_classifier.hpp:116-164): captured
+    iff R(r) = (r^2 + a^2 - a xi)^2 - Delta (eta + (xi - a)^2) has no real root
+    outside r_+. Returns (captured, margin) where margin = min R(r)/r^4 over
+    r in [r_+, 50 M] (0 on the critical curve)."""
+    K = eta + (xi - a) ** 2
+    coeffs = [1.0, 0.0, 2 * a * a - 2 * a * xi - K, 2 * M * K, (a * a - a * xi) ** 2 - a * a * K]
+    rp = M + math.sqrt(M * M - a * a)
+    roots = np.roots(coeffs)
+    real = roots[np.abs(roots.imag) < 1e-9].real
+    captured = not np.any(real > rp)
+    rr = np.linspace(rp, 50.0, 4000)
+    R = np.polyval(coeffs, rr) / rr ** 4
+    return captured, float(np.min(np.abs(R)))
+
+
+@pytest.mark.parametrize("a,incl", [(0.0, 60.0), (0.9, 60.0), (0.998, 85.0), (0.6, 17.0)])
+def test_classes_match_carter_classifier(a, incl):
+    met = rtsim.kerr_metric(1.0, a)
+    tm = rtsim.default_termination(escape_radius=1000.0, disk_enabled=False)
+    n = 24
+    cam = rtsim.default_camera(1000.0, math.radians(incl), 0.0, math.radians(1.6), n, n)
+    res = O.trace_camera(cam, met, rtsim.default_integrator(), tm)
+    checked = 0
+    for i in range(n * n):
+        ok, y0, _ = O.camera_state(cam, met, tm, i % n, i // n)
+        E, Lz, Q = carter(y0, a)
+        cap, margin = carter_captured(Lz / E, Q / (E * E), a)
+        if margin < 1e-4:
+            continue  # within reach of the critical curve
+        checked += 1
+        assert (res.klass[i] == 1) == cap, (i, Lz, Q, res.klass[i])
+    assert checked > n * n * 0.8
+    assert (res.klass == 1).sum() > 20 and (res.klass == 2).sum() > 20
