@@ -42,6 +42,7 @@ R_OBS = 1000.0
 FLOP_RHS = 178
 FLOP_ATTEMPT = 432
 PEAK_FP64_TFLOPS = 78.6
+METRIC = "rays/s Kerr 2048\u00b2 mixed vs FP64 at \u22641e-9 rel err; p50/p99 latency @QPS"
 
 
 def frame_requests(side: int, precision: int):
@@ -180,6 +181,20 @@ def precision_report(side: int, switch_radius: float):
     return rep
 
 
+def copy_probe(mib: int):
+    """Three plain device-to-device copies of a known size: what FETCH_SIZE
+    and WRITE_SIZE read for them calibrates the counters against bytes."""
+    import torch
+
+    src = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": mib << 20, "copy_bytes_written": mib << 20, "copies": 3}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +216,8 @@ def main() -> int:
     ap.add_argument("--precision-report", type=int, default=0, metavar="SIDE",
                     help="print a mixed-vs-FP64 error report at this camera side (e.g. 4096) and exit")
     ap.add_argument("--switch-radius", type=float, default=100.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run three device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this tracer runs on one GPU; --gpus {args.gpus} is not supported "
@@ -209,11 +226,21 @@ def main() -> int:
     if args.precision_report:
         print(json.dumps(precision_report(args.precision_report, args.switch_radius)), flush=True)
         return 0
+    if args.copy_probe:
+        print(json.dumps(copy_probe(args.copy_probe)), flush=True)
+        return 0
     prec = rtsim.RT_PRECISION_MIXED if args.precision == "mixed" else rtsim.RT_PRECISION_FP64
     reqs = frame_requests(args.side, prec)
     if args.workload == "uneven512":
         reqs = [(r.camera, r.metric, r.integrator, r.termination) for r in uneven_requests()]
     ctx = rtsim.Context(0)
+    tuned = None
+    if args.workload == "uneven512" and args.path == "batch":
+        # autotune the work-queue launch shape on every 16th request
+        best = ctx.autotune([rtsim.make_request(*r) for r in reqs[::16]])
+        tuned = {"block": best.block, "waves": best.waves, "refill_min": best.refill_min,
+                 "lane_refill": best.lane_refill}
+        print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
     print(f"bench: {len(reqs)} frames of {args.side}^2, precision {args.precision}, "
           f"warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
     batch = [rtsim.make_request(*r) for r in reqs]
@@ -256,7 +283,7 @@ def main() -> int:
     avg_kernel_s = kernel_ms / launches / 1e3
     achieved = flops / launches / avg_kernel_s / 1e12
     out = {
-        "metric": "rays/s Kerr 2048^2 (4 spins, i=60 deg) DP45 rtol 1e-10",
+        "metric": METRIC,
         "value": value,
         "unit": "rays/s",
         "n_gpus": 1,
@@ -273,7 +300,8 @@ def main() -> int:
                    "spins": list(SPINS), "precision": args.precision, "rays_per_step": rays // args.steps,
                    "mean_accepted_steps": acc / rays, "mean_rejected_steps": rej / rays,
                    "rhs_evals_per_ray": rhs / rays, "path": args.path, "kernel_ms_per_launch": kernel_ms / launches,
-                   "launches_per_step": launches // args.steps, "block": shape[0], "grid": shape[1]},
+                   "launches_per_step": launches // args.steps, "block": shape[0], "grid": shape[1],
+                   "autotuned": tuned},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
     }

@@ -13,8 +13,72 @@
 #include "relativistic/metrics/kerr_schild.hpp"
 #include "relativistic/optics/disk_thermal_profile.hpp"
 #include "relativistic/render/accretion_disk/accretion_disk_model.hpp"
+#include "relativistic/render/software_compute_engine.hpp"
+
+#include <span>
+#include <vector>
+
+#include "../include/rt_tracer.h"
 
 using namespace Relativistic;
+
+namespace {
+
+// The renderer's superposition helpers (build_multi_source_data, ms_rhs,
+// ms_initial_momentum) are private static members of SoftwareComputeEngine.
+// An explicit template instantiation may name private members, and a friend
+// defined by that instantiation hands the member pointer out; nothing of the
+// reference is restated here.
+template <class Tag, auto Member>
+struct Expose {
+    friend auto exposed(Tag) { return Member; }
+};
+struct BuildTag {};
+struct MsRhsTag {};
+struct MsMomentumTag {};
+auto exposed(BuildTag);
+auto exposed(MsRhsTag);
+auto exposed(MsMomentumTag);
+template struct Expose<BuildTag, &Render::SoftwareComputeEngine::build_multi_source_data>;
+template struct Expose<MsRhsTag, &Render::SoftwareComputeEngine::ms_rhs>;
+template struct Expose<MsMomentumTag, &Render::SoftwareComputeEngine::ms_initial_momentum>;
+
+// rt_metric (src[0] = central hole with dimensionless spin, src[1..] = bodies)
+// -> the renderer's push constants and GpuBodyData with preset_3d == 8, the
+// field mapping of the C ABI. Disk radii map to inner_radius_scale = r_in /
+// ISCO (1 when r_in = 0) and outer_radius_mass_units = r_out / m.
+auto build_ms(const rt_metric* mt) {
+    Render::GpuCameraPushConstants pc{};
+    pc.metric_type = 3U;
+    const rt_source& c = mt->src[0];
+    pc.metric_mass = c.mass;
+    pc.metric_spin = c.spin * c.mass;
+    auto disk = [](Render::GpuDiskProfile& d, const rt_source& s, double m, double a) {
+        d.enabled = s.disk_enabled ? 1.0f : 0.0f;
+        const double isco = Optics::DiskThermalProfile::kerr_isco_radius(m, a);
+        d.inner_radius_scale = s.disk_inner > 0 ? static_cast<float>(s.disk_inner / isco) : 1.0f;
+        d.outer_radius_mass_units = static_cast<float>((s.disk_outer > 0 ? s.disk_outer : 20.0 * m) / m);
+    };
+    {
+        const double m = std::max(c.mass, 1e-4);
+        disk(pc.primary_disk, c, m, std::clamp(c.spin * c.mass, -0.999 * m, 0.999 * m));
+    }
+    std::vector<Render::GpuBodyData> bodies(mt->n_sources > 1 ? mt->n_sources - 1 : 0);
+    for (size_t i = 0; i < bodies.size(); ++i) {
+        const rt_source& s = mt->src[i + 1];
+        Render::GpuBodyData& b = bodies[i];
+        b.preset_3d = 8U;
+        b.position = {s.position[0], s.position[1], s.position[2], 0.0};
+        b.mass = s.mass;
+        b.spin_parameter = s.spin;
+        b.spin_axis = {s.axis[0], s.axis[1], s.axis[2], 0.0};
+        const double m = std::max(s.mass, 1e-9);
+        disk(b.disk, s, m, std::clamp(s.spin, -0.9999, 0.9999) * m);
+    }
+    return exposed(BuildTag{})(pc, std::span<const Render::GpuBodyData>(bodies.data(), bodies.size()));
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -137,6 +201,47 @@ int ref_rk45_integrate(const double x0[4], const double u0[4], double M, double 
     *acc = integ.statistics().accepted_steps;
     *rej = integ.statistics().rejected_steps;
     return 0;
+}
+
+// render/software_compute_engine.hpp:2738-2798 (build_multi_source_data), in the
+// 17-doubles-per-source layout of rto_ms_table. Returns the source count.
+int ref_ms_table(const rt_metric* mt, double* out) {
+    const auto ms = build_ms(mt);
+    for (size_t i = 0; i < ms.count; ++i) {
+        double* o = out + 17 * i;
+        for (int j = 0; j < 3; ++j) {
+            o[j] = ms.position_mass[i][j];
+            o[6 + j] = ms.frame_x_horizon[i][j];
+            o[9 + j] = ms.frame_y_equatorial[i][j];
+            o[12 + j] = ms.axis_spin[i][j];
+        }
+        o[3] = ms.position_mass[i][3];
+        o[4] = ms.axis_spin[i][3];
+        o[5] = ms.frame_x_horizon[i][3];
+        o[15] = ms.disk_range[i][0];
+        o[16] = ms.disk_range[i][1];
+    }
+    return static_cast<int>(ms.count);
+}
+
+// render/software_compute_engine.hpp:2857-2870 (ms_rhs, p_t = -1)
+void ref_ms_rhs(const rt_metric* mt, const double x[3], const double p[3], double dx[3], double dp[3]) {
+    const auto ms = build_ms(mt);
+    std::array<double, 3> ox{}, op{};
+    exposed(MsRhsTag{})(ms, {x[0], x[1], x[2]}, {p[0], p[1], p[2]}, ox, op);
+    for (int i = 0; i < 3; ++i) {
+        dx[i] = ox[i];
+        dp[i] = op[i];
+    }
+}
+
+// render/software_compute_engine.hpp:2919-2964 (ms_initial_momentum)
+void ref_ms_initial_momentum(const rt_metric* mt, const double x[3], const double outward[3], double p[3]) {
+    const auto ms = build_ms(mt);
+    const auto q = exposed(MsMomentumTag{})(ms, {x[0], x[1], x[2]}, {outward[0], outward[1], outward[2]});
+    p[0] = q[0];
+    p[1] = q[1];
+    p[2] = q[2];
 }
 
 }  // extern "C"

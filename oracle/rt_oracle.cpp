@@ -515,63 +515,15 @@ RayOut trace_ray(const Spacetime& st, const Integ& ig, const double y_init[8], d
     return o;
 }
 
-// -------------------------------------------------------------- camera
-// Pixel -> initial state, restating dispatch_fp64_scalar (software_compute_engine.hpp:3843-3938),
-// CameraProjector::compute_pinhole_ray (observer/camera_projections.hpp:27-41) and the
-// set-up of trace_kerr_schild_photon (:1634-1652) / trace_multi_source (:3054-3066).
-// Returns false when the observer sits inside a horizon (captured at once).
-bool camera_ray(const Spacetime& st, const rt_camera* cam, uint32_t px, uint32_t py, double y[8], double* xi) {
-    const double W = cam->width, Hh = cam->height;
-    const double aspect = W / Hh;
-    const double v = 1.0 - (static_cast<double>(py) + 0.5) / Hh * 2.0;
-    const double u = ((static_cast<double>(px) + 0.5) / W * 2.0 - 1.0) * aspect;
-    const double tan_half = std::tan(cam->fov_y * 0.5);
-    const double n1 = 1.0, n2 = v * tan_half, n3 = u * tan_half;
-    const double len = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
-    const double inv_len = (len > 0.0) ? (1.0 / len) : 1.0;
-    const double nl[3] = {n1 * inv_len, n2 * inv_len, n3 * inv_len};
-    const double d[3] = {nl[0] * cam->forward[0] + nl[2] * cam->right[0] + nl[1] * cam->up[0],
-                         nl[0] * cam->forward[1] + nl[2] * cam->right[1] + nl[1] * cam->up[1],
-                         nl[0] * cam->forward[2] + nl[2] * cam->right[2] + nl[1] * cam->up[2]};
-    const double r_obs = max(cam->observer[1], st.rh * 1.02);
-    const double th = std::clamp(cam->observer[2], 0.001, kPi - 0.001);
-    const double ph = cam->observer[3];
-    const double st_ = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
-    const double n_r = d[0] * (st_ * cp) + d[1] * (st_ * sp) + d[2] * ct;
-    const double n_th = d[0] * (ct * cp) + d[1] * (ct * sp) + d[2] * (-st_);
-    const double n_ph = d[0] * (-sp) + d[1] * cp + d[2] * 0.0;
-    const double outward[3] = {n_r * st_ * cp + n_th * ct * cp - n_ph * sp, n_r * st_ * sp + n_th * ct * sp + n_ph * cp,
-                               n_r * ct - n_th * st_};
-    y[0] = 0.0;
-    y[4] = -1.0;
-    if (!st.multi) {
-        ks_position_from_bl(r_obs, th, ph, st.a, y + 1);
-        ks_initial_momentum(y + 1, st.M, st.a, outward, y + 5);
-        *xi = y[1] * y[6] - y[2] * y[5];
-        return true;
-    }
-    // multi-source: flat spherical position, superposed-metric frame.
-    y[1] = r_obs * st_ * cp;
-    y[2] = r_obs * st_ * sp;
-    y[3] = r_obs * ct;
-    *xi = 0.0;
-    for (int i = 0; i < st.n; ++i) {
-        const Source& s = st.s[i];
-        const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
-        double l[3];
-        to_local(s, dx, l);
-        const double a2 = s.a * s.a;
-        const double diff = (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]) - a2;
-        const double r2 = max(0.5 * (diff + sqrt(diff * diff + 4.0 * a2 * l[2] * l[2])), 1e-24);
-        const double r = sqrt(r2);
-        if (r <= s.rh * 1.0001 || (r2 - 2.0 * s.m * r + a2) <= 0.0) return false;
-    }
-    // ms_initial_momentum (software_compute_engine.hpp:2919-2964)
+// Restates ms_initial_momentum (software_compute_engine.hpp:2919-2964): the
+// superposed metric's Gram-Schmidt frame at x, and the covariant spatial
+// momentum of the backward ray along -outward, normalised to p_t = -1.
+static void ms_initial_momentum(const Spacetime& st, const double x[3], const double outward[3], double p[3]) {
     double pot[RT_MAX_SOURCES], dir[RT_MAX_SOURCES][3];
     double psum = 0.0;
     for (int i = 0; i < st.n; ++i) {
         const Source& s = st.s[i];
-        const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
+        const double dx[3] = {x[0] - s.c[0], x[1] - s.c[1], x[2] - s.c[2]};
         double l[3];
         to_local(s, dx, l);
         const KSField<double> f = ks_field(l[0], l[1], l[2], s.m, s.a);
@@ -628,9 +580,63 @@ bool camera_ray(const Spacetime& st, const rt_camera* cam, uint32_t px, uint32_t
         mom[2] += 2.0 * pot[i] * lv * dir[i][2];
     }
     const double inv_pt = 1.0 / max(-tc, 1e-12);
-    y[5] = mom[0] * inv_pt;
-    y[6] = mom[1] * inv_pt;
-    y[7] = mom[2] * inv_pt;
+    p[0] = mom[0] * inv_pt;
+    p[1] = mom[1] * inv_pt;
+    p[2] = mom[2] * inv_pt;
+}
+
+// -------------------------------------------------------------- camera
+// Pixel -> initial state, restating dispatch_fp64_scalar (software_compute_engine.hpp:3843-3938),
+// CameraProjector::compute_pinhole_ray (observer/camera_projections.hpp:27-41) and the
+// set-up of trace_kerr_schild_photon (:1634-1652) / trace_multi_source (:3054-3066).
+// Returns false when the observer sits inside a horizon (captured at once).
+bool camera_ray(const Spacetime& st, const rt_camera* cam, uint32_t px, uint32_t py, double y[8], double* xi) {
+    const double W = cam->width, Hh = cam->height;
+    const double aspect = W / Hh;
+    const double v = 1.0 - (static_cast<double>(py) + 0.5) / Hh * 2.0;
+    const double u = ((static_cast<double>(px) + 0.5) / W * 2.0 - 1.0) * aspect;
+    const double tan_half = std::tan(cam->fov_y * 0.5);
+    const double n1 = 1.0, n2 = v * tan_half, n3 = u * tan_half;
+    const double len = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
+    const double inv_len = (len > 0.0) ? (1.0 / len) : 1.0;
+    const double nl[3] = {n1 * inv_len, n2 * inv_len, n3 * inv_len};
+    const double d[3] = {nl[0] * cam->forward[0] + nl[2] * cam->right[0] + nl[1] * cam->up[0],
+                         nl[0] * cam->forward[1] + nl[2] * cam->right[1] + nl[1] * cam->up[1],
+                         nl[0] * cam->forward[2] + nl[2] * cam->right[2] + nl[1] * cam->up[2]};
+    const double r_obs = max(cam->observer[1], st.rh * 1.02);
+    const double th = std::clamp(cam->observer[2], 0.001, kPi - 0.001);
+    const double ph = cam->observer[3];
+    const double st_ = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
+    const double n_r = d[0] * (st_ * cp) + d[1] * (st_ * sp) + d[2] * ct;
+    const double n_th = d[0] * (ct * cp) + d[1] * (ct * sp) + d[2] * (-st_);
+    const double n_ph = d[0] * (-sp) + d[1] * cp + d[2] * 0.0;
+    const double outward[3] = {n_r * st_ * cp + n_th * ct * cp - n_ph * sp, n_r * st_ * sp + n_th * ct * sp + n_ph * cp,
+                               n_r * ct - n_th * st_};
+    y[0] = 0.0;
+    y[4] = -1.0;
+    if (!st.multi) {
+        ks_position_from_bl(r_obs, th, ph, st.a, y + 1);
+        ks_initial_momentum(y + 1, st.M, st.a, outward, y + 5);
+        *xi = y[1] * y[6] - y[2] * y[5];
+        return true;
+    }
+    // multi-source: flat spherical position, superposed-metric frame.
+    y[1] = r_obs * st_ * cp;
+    y[2] = r_obs * st_ * sp;
+    y[3] = r_obs * ct;
+    *xi = 0.0;
+    for (int i = 0; i < st.n; ++i) {
+        const Source& s = st.s[i];
+        const double dx[3] = {y[1] - s.c[0], y[2] - s.c[1], y[3] - s.c[2]};
+        double l[3];
+        to_local(s, dx, l);
+        const double a2 = s.a * s.a;
+        const double diff = (l[0] * l[0] + l[1] * l[1] + l[2] * l[2]) - a2;
+        const double r2 = max(0.5 * (diff + sqrt(diff * diff + 4.0 * a2 * l[2] * l[2])), 1e-24);
+        const double r = sqrt(r2);
+        if (r <= s.rh * 1.0001 || (r2 - 2.0 * s.m * r + a2) <= 0.0) return false;
+    }
+    ms_initial_momentum(st, y + 1, outward, y + 5);
     return true;
 }
 
@@ -803,6 +809,38 @@ int rto_project(double y[8], const rt_metric* mt, double tol) {
 
 void rto_initial_momentum(const double x[3], double M, double a, const double outward[3], double p[3]) {
     ks_initial_momentum(x, M, a, outward, p);
+}
+
+// Superposition source table (build_multi_source_data) as 17 doubles per
+// source: centre[3], m, a, r_+, frame x[3], frame y[3], axis[3], disk in, disk
+// out (-1, -1 when the source's disk is off). Returns the source count.
+int rto_ms_table(const rt_metric* mt, const rt_termination* tm, double* out) {
+    const Spacetime st = make_spacetime(mt, tm);
+    if (!st.multi) return 0;
+    for (int i = 0; i < st.n; ++i) {
+        const Source& s = st.s[i];
+        double* o = out + 17 * i;
+        for (int j = 0; j < 3; ++j) {
+            o[j] = s.c[j];
+            o[6 + j] = s.fx[j];
+            o[9 + j] = s.fy[j];
+            o[12 + j] = s.ax[j];
+        }
+        o[3] = s.m;
+        o[4] = s.a;
+        o[5] = s.rh;
+        o[15] = s.disk_on ? s.disk_in : -1.0;
+        o[16] = s.disk_on ? s.disk_out : -1.0;
+    }
+    return st.n;
+}
+
+void rto_ms_initial_momentum(const rt_metric* mt, const double x[3], const double outward[3], double p[3]) {
+    rt_termination tm{};
+    tm.horizon_eps = 1e-6;
+    tm.escape_radius = 1000;
+    const Spacetime st = make_spacetime(mt, &tm);
+    ms_initial_momentum(st, x, outward, p);
 }
 
 void rto_ks_from_bl(double r, double th, double ph, double a, double out[3]) { ks_position_from_bl(r, th, ph, a, out); }

@@ -2,8 +2,10 @@
 // autotuner (rt_*_launch_shape, rt_autotune), and the serving loop (rt_server_*).
 //
 // A batch is laid out as one work queue: every single-Kerr request becomes a
-// contiguous range of queue indices (8x8-tile order for cameras) served by the
-// persistent k_trace_queue kernel; superposition requests run as their own
+// contiguous range of queue indices (8x8 tiles, or the rays of those tiles in
+// tile order) served by one persistent kernel, k_trace_tiles (a wave takes a
+// whole tile) or k_trace_queue (idle lanes refill ray by ray); superposition
+// requests run as their own
 // k_trace_multi kernels after it. The launch sequence of a batch is captured
 // into a hipGraph and replayed while the batch layout (pointers, sizes,
 // by-value kernel arguments, launch shape) stays the same.
@@ -547,7 +549,7 @@ static bool shape_valid(const rt_launch_shape* s) {
 rt_status rt_set_launch_shape(rt_context* ctx, const rt_launch_shape* plain, const rt_launch_shape* queue) {
     if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
     if ((plain && !shape_valid(plain)) || (queue && !shape_valid(queue)))
-        return fail(RT_ERR_INVALID_ARG, "launch shape: block in {64,128,192,256}, waves in {1,2}, refill_min 1..64");
+        return fail(RT_ERR_INVALID_ARG, "launch shape: block in {64,128,192,256}, waves in {1,2}, refill_min 1..64, lane_refill in {0,1}");
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (plain) ctx->shape = LaunchShape{plain->block, plain->waves, plain->refill_min, plain->lane_refill};
     if (queue) ctx->qshape = LaunchShape{queue->block, queue->waves, queue->refill_min, queue->lane_refill};

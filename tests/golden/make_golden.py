@@ -1,6 +1,8 @@
 """Writes tests/golden/ref_vectors.json: inputs and outputs of the reference's
 own functions (KS field, Hamiltonian RHS, initial momentum, coordinate maps,
-ISCO, disk redshift, and end points of its Christoffel-based RK45 integrator),
+ISCO, disk redshift, end points of its Christoffel-based RK45 integrator, and
+the renderer's superposition source table, right-hand side and initial
+momentum),
 computed through oracle/_ref/libref.so, which oracle/Makefile compiles from
 the reference headers. Run after `make -C oracle`; the fixture lets the oracle
 be checked where the reference is not available."""
@@ -72,6 +74,37 @@ def main():
             ref.ref_rk45_integrate(ptr(yf[:4].copy()), ptr(u), 1.0, a, 40.0, 1e-12, 1e-14, 0.5, ptr(xo), ptr(uo),
                                    C.byref(acc), C.byref(rej))
             out["rk45"].append({"a": a, "y0": yf.tolist(), "L": 40.0, "x_end": xo.tolist(), "u_end": uo.tolist()})
+    # superpositions of 2..16 sources (separate stream: the vectors above stay as they were)
+    rng = np.random.default_rng(20261204)
+    out["ms_table"], out["ms_rhs"], out["ms_momentum"] = [], [], []
+    for mi in range(16):
+        k = int(rng.integers(2, 17))
+        src = [{"mass": float(rng.uniform(0.5, 2.0)), "spin": float(rng.uniform(-0.99, 0.99)),
+                "disk_enabled": bool(rng.random() < 0.7)}]
+        for _ in range(k - 1):
+            src.append({"mass": float(rng.uniform(0.1, 1.0)), "spin": float(rng.uniform(-0.95, 0.95)),
+                        "position": [float(v) for v in rng.uniform(-25, 25, size=3)],
+                        "axis": [float(v) for v in rng.normal(size=3)],
+                        "disk_enabled": bool(rng.random() < 0.7)})
+        met = rtsim.multi_metric(src)
+        t = np.zeros(17 * 16)
+        n = ref.ref_ms_table(C.byref(met), ptr(t))
+        out["ms_table"].append({"sources": src, "table": t[:17 * n].tolist()})
+        for _ in range(4):
+            c = np.array(src[int(rng.integers(0, k))].get("position", [0.0, 0.0, 0.0]))
+            x = c + rng.normal(size=3) * float(rng.choice([3.0, 10.0, 100.0]))
+            p = rng.normal(size=3)
+            dx, dp = np.zeros(3), np.zeros(3)
+            ref.ref_ms_rhs(C.byref(met), ptr(x), ptr(p), ptr(dx), ptr(dp))
+            out["ms_rhs"].append({"metric": mi, "x": x.tolist(), "p": p.tolist(), "dx": dx.tolist(),
+                                  "dp": dp.tolist()})
+        x = rng.normal(size=3)
+        x *= float(rng.uniform(40.0, 1000.0)) / np.linalg.norm(x)
+        o = rng.normal(size=3)
+        o /= np.linalg.norm(o)
+        q = np.zeros(3)
+        ref.ref_ms_initial_momentum(C.byref(met), ptr(x), ptr(o), ptr(q))
+        out["ms_momentum"].append({"metric": mi, "x": x.tolist(), "outward": o.tolist(), "p": q.tolist()})
     path = os.path.join(HERE, "ref_vectors.json")
     with open(path, "w") as fh:
         json.dump(out, fh, indent=0)

@@ -54,6 +54,9 @@ def oracle():
         lib.rto_constraint.restype = D
         lib.rto_project.argtypes = [PD, P(rt_metric), D]
         lib.rto_initial_momentum.argtypes = [PD, D, D, PD, PD]
+        lib.rto_ms_table.argtypes = [P(rt_metric), P(rt_termination), PD]
+        lib.rto_ms_table.restype = C.c_int
+        lib.rto_ms_initial_momentum.argtypes = [P(rt_metric), PD, PD, PD]
         lib.rto_ks_from_bl.argtypes = [D, D, D, D, PD]
         lib.rto_bl_from_ks.argtypes = [PD, D, PD]
         lib.rto_isco.argtypes = [D, D]
@@ -95,6 +98,10 @@ def ref():
         lib.ref_ks_metric.argtypes = [PD, D, D, PD, PD]
         lib.ref_rk45_integrate.argtypes = [PD, PD, D, D, D, D, D, D, PD, PD, P(C.c_uint64), P(C.c_uint64)]
         lib.ref_rk45_integrate.restype = C.c_int
+        lib.ref_ms_table.argtypes = [P(rt_metric), PD]
+        lib.ref_ms_table.restype = C.c_int
+        lib.ref_ms_rhs.argtypes = [P(rt_metric), PD, PD, PD, PD]
+        lib.ref_ms_initial_momentum.argtypes = [P(rt_metric), PD, PD, PD]
         _REF = lib
     return _REF
 

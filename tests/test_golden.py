@@ -70,3 +70,34 @@ def test_rk45_end_points():
                                     out.ctypes.data_as(PD)) > 0
         xe = np.array(v["x_end"])
         assert np.abs(out[:4] - xe).max() < 1e-10 * max(1.0, np.abs(xe).max())
+
+
+def _metric(i):
+    return rtsim.multi_metric(GOLD["ms_table"][i]["sources"])
+
+
+def test_ms_table():
+    tm = rtsim.default_termination()
+    for v in GOLD["ms_table"]:
+        met = rtsim.multi_metric(v["sources"])
+        t = np.zeros(17 * 16)
+        n = O.oracle().rto_ms_table(C.byref(met), C.byref(tm), t.ctypes.data_as(PD))
+        assert n == len(v["sources"]) and np.array_equal(t[:17 * n], np.array(v["table"]))
+
+
+def test_ms_rhs():
+    # the oracle's superposition RHS sums the same per-source terms in another
+    # algebraic order than the renderer's ks_source_terms: equal to rounding
+    for v in GOLD["ms_rhs"]:
+        d = O.rhs(np.array([0.0, *v["x"], -1.0, *v["p"]]), _metric(v["metric"]))
+        ref = np.array(v["dx"] + v["dp"])
+        got = np.concatenate([d[1:4], d[5:8]])
+        assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_ms_initial_momentum():
+    for v in GOLD["ms_momentum"]:
+        p = np.zeros(3)
+        O.oracle().rto_ms_initial_momentum(C.byref(_metric(v["metric"])), ptr(np.array(v["x"])),
+                                           ptr(np.array(v["outward"])), p.ctypes.data_as(PD))
+        assert np.array_equal(p, np.array(v["p"]))

@@ -292,3 +292,124 @@ def test_classes_match_carter_classifier(a, incl):
         assert (res.klass[i] == 1) == cap, (i, Lz, Q, res.klass[i])
     assert checked > n * n * 0.8
     assert (res.klass == 1).sum() > 20 and (res.klass == 2).sum() > 20
+
+
+# ------------------------------------------------------------ superposition
+def rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q @ np.diag(np.sign(np.diag(r)))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def sources_config(rng, k):
+    src = [{"mass": float(rng.uniform(0.5, 1.5)), "spin": float(rng.uniform(-0.9, 0.9)), "axis": (0.0, 0.0, 1.0)}]
+    for _ in range(k - 1):
+        ax = rng.normal(size=3)
+        src.append({"mass": float(rng.uniform(0.1, 1.0)), "spin": float(rng.uniform(-0.95, 0.95)),
+                    "position": tuple(rng.uniform(-20, 20, size=3)), "axis": tuple(ax / np.linalg.norm(ax))})
+    return src
+
+
+def test_superposition_rotation_covariance():
+    """Rotating every source centre and spin axis and the state by R rotates
+    the right-hand side by R and leaves p_t's derivative and dt/dl unchanged:
+    pins the per-source frames and the local <-> world maps (a transposed or
+    mis-signed frame breaks it for general axes)."""
+    rng = np.random.default_rng(21)
+    for _ in range(60):
+        src = sources_config(rng, int(rng.integers(2, 9)))
+        R = rotation(rng)
+        rot = [dict(s, position=tuple(R @ np.array(s.get("position", (0.0, 0.0, 0.0)))),
+                    axis=tuple(R @ np.array(s["axis"]))) for s in src]
+        j = int(rng.integers(0, len(src)))
+        x = np.array(src[j].get("position", (0.0, 0.0, 0.0))) + rng.normal(size=3) * 4.0
+        p = rng.normal(size=3)
+        y = np.array([0.0, *x, -1.0, *p])
+        yr = np.array([0.0, *(R @ x), -1.0, *(R @ p)])
+        d = O.rhs(y, rtsim.multi_metric(src))
+        dr = O.rhs(yr, rtsim.multi_metric(rot))
+        scale = np.abs(d).max()
+        assert abs(dr[0] - d[0]) <= 1e-12 * scale and dr[4] == d[4] == 0.0
+        assert np.abs(dr[1:4] - R @ d[1:4]).max() <= 1e-12 * scale
+        assert np.abs(dr[5:8] - R @ d[5:8]).max() <= 1e-12 * scale
+        assert abs(O.constraint(yr, rtsim.multi_metric(rot)) - O.constraint(y, rtsim.multi_metric(src))) \
+            <= 1e-12 * (1.0 + p @ p)
+
+
+def test_superposition_spin_axis_flip():
+    """A body of spin s about n is the same hole as spin -s about -n."""
+    rng = np.random.default_rng(22)
+    for _ in range(60):
+        src = sources_config(rng, int(rng.integers(2, 6)))
+        flip = [src[0]] + [dict(s, spin=-s["spin"], axis=tuple(-np.array(s["axis"]))) for s in src[1:]]
+        x = np.array(src[-1]["position"]) + rng.normal(size=3) * 3.0
+        y = np.array([0.0, *x, -1.0, *rng.normal(size=3)])
+        d, df = O.rhs(y, rtsim.multi_metric(src)), O.rhs(y, rtsim.multi_metric(flip))
+        assert np.abs(df - d).max() <= 1e-12 * np.abs(d).max()
+
+
+@pytest.mark.parametrize("a", [0.0, 0.9])
+def test_superposition_reduces_to_kerr(a):
+    """A central hole plus a body of negligible mass far away: the same
+    right-hand side as the single Kerr hole (to the body's ~1e-9 M / 1e6 M
+    field), and the same rays: identical classes, disk hits at the same
+    points, escapes within the escape tests' own difference (Euclidean
+    |x| = r_esc against KS r = r_esc: |x| - r = a^2 sin^2(theta) / 2r).
+    Redshifts are not compared: the superposition path takes xi = local L_z
+    over |p| at the crossing (ms_disk_crossing, software_compute_engine.hpp:3021-3042),
+    the single-hole path L_z / E of the ray (:1650)."""
+    single = rtsim.kerr_metric(1.0, a)
+    multi = rtsim.multi_metric([{"mass": 1.0, "spin": a},
+                                {"mass": 1e-9, "spin": 0.0, "position": (1e6, 0.0, 0.0), "disk_enabled": False}])
+    rng = np.random.default_rng(23)
+    for _ in range(100):
+        x = rng.normal(size=3)
+        x *= rng.uniform(2.5, 300.0) / np.linalg.norm(x)
+        y = np.array([0.0, *x, -1.0, *rng.normal(size=3)])
+        d1, d2 = O.rhs(y, single), O.rhs(y, multi)
+        assert np.abs(d2 - d1).max() <= 1e-13 * np.abs(d1).max() + 1e-14
+    cam = rtsim.default_camera(60.0, math.radians(75), 0.0, math.radians(30), 9, 9)
+    r_esc = 200.0
+    tm = rtsim.default_termination(escape_radius=r_esc)
+    xs, ps = [], []
+    for i in range(81):
+        ok, y0, _ = O.camera_state(cam, single, tm, i % 9, i // 9)
+        xs.append(y0[:4])
+        ps.append(y0[4:])
+    xs, ps = np.array(xs).T.copy(), np.array(ps).T.copy()
+    r1 = O.trace_rays(xs, ps, single, rtsim.default_integrator(), tm)
+    r2 = O.trace_rays(xs, ps, multi, rtsim.default_integrator(), tm)
+    assert np.array_equal(r1.klass, r2.klass)
+    assert {1, 2, 4} <= set(r1.klass.tolist())
+    disk, esc = r1.klass == 4, r1.klass == 2
+    assert np.abs(r2.x[1:, disk] - r1.x[1:, disk]).max() <= 1e-9 * 20.0
+    assert np.abs(r2.x[1:, esc] - r1.x[1:, esc]).max() <= a * a / (2 * r_esc) * 1.01 + 1e-9 * r_esc
+
+
+def test_superposition_mirror_camera():
+    """Two holes placed as mirror images through the plane x = 0: centres
+    (+-9, 0, 0), spin 0.7 about n and about its mirror image (n_x, -n_y, -n_z)
+    (spin is an axial vector), a light central hole without disk, and an
+    observer on the y axis whose right vector is -x: pixel (W-1-i, j) sees the
+    mirror image of pixel (i, j), with the same class and redshift. (The tilt
+    keeps the two disks out of one plane: coplanar disks would tie and go to
+    the lower source index.)"""
+    n1 = np.array([0.3, 0.2, 1.0]) / math.sqrt(1.13)
+    met = rtsim.multi_metric([
+        {"mass": 0.2, "spin": 0.0, "disk_enabled": False},
+        {"mass": 0.8, "spin": 0.7, "position": (9.0, 0.0, 0.0), "axis": tuple(n1)},
+        {"mass": 0.8, "spin": 0.7, "position": (-9.0, 0.0, 0.0), "axis": (n1[0], -n1[1], -n1[2])}])
+    n = 16
+    cam = rtsim.default_camera(80.0, math.radians(70), math.radians(90), math.radians(40), n, n)
+    assert np.allclose(cam.right[:], [-1.0, 0.0, 0.0], atol=1e-15)
+    res = O.trace_camera(cam, met, rtsim.default_integrator(), rtsim.default_termination(escape_radius=300.0))
+    k = res.klass.reshape(n, n)
+    assert np.array_equal(k, k[:, ::-1])
+    assert {1, 2, 4} <= set(res.klass.tolist())
+    X = res.x.reshape(4, n, n)
+    ok = k != 1
+    assert np.abs(X[1][ok] + X[1][:, ::-1][ok]).max() < 1e-6
+    assert np.abs(X[2][ok] - X[2][:, ::-1][ok]).max() < 1e-6
+    assert np.abs(X[3][ok] - X[3][:, ::-1][ok]).max() < 1e-6
+    g = res.g.reshape(n, n)
+    assert np.abs(g - g[:, ::-1]).max() < 1e-7
