@@ -103,11 +103,11 @@ def serving_requests(n: int, seed: int):
 
 def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
     """Poisson arrivals at a fixed QPS into the serving loop; latency is
-    submit -> results on the host."""
+    submit -> results on the host. Reported overall and per request kind."""
     reqs = serving_requests(n, seed)
     rng = np.random.default_rng(seed + 1)
     arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
-    lat = []
+    lat, kind = [], []
     rays = 0
     with rtsim.Server(0, hbm_budget_bytes=16 << 30, max_inflight=256) as srv:
         pending = {}
@@ -116,19 +116,26 @@ def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
         while i < n or pending:
             now = time.perf_counter() - t0
             while i < n and arrivals[i] <= now:
-                pending[srv.submit(reqs[i])] = (time.perf_counter(), rtsim.n_rays_of(reqs[i].camera))
+                pending[srv.submit(reqs[i])] = (time.perf_counter(), rtsim.n_rays_of(reqs[i].camera),
+                                                reqs[i].metric.n_sources >= 2)
                 i += 1
             for rid in [r for r in pending if srv.poll(r)]:
-                ts, nr = pending.pop(rid)
+                ts, nr, multi = pending.pop(rid)
                 srv.take(rid)
                 lat.append(time.perf_counter() - ts)
+                kind.append(multi)
                 rays += nr
             time.sleep(0.0002)
         wall = time.perf_counter() - t0
-    lat_ms = np.array(lat) * 1e3
+    lat_ms, kind = np.array(lat) * 1e3, np.array(kind)
+
+    def pct(v):
+        return {"p50_ms": float(np.percentile(v, 50)), "p99_ms": float(np.percentile(v, 99)), "n": int(v.size)}
+
     return {"qps": qps, "requests": n, "p50_ms": float(np.percentile(lat_ms, 50)),
             "p99_ms": float(np.percentile(lat_ms, 99)), "mean_ms": float(lat_ms.mean()),
-            "rays_per_s": rays / wall, "multi_source_fraction": 0.3,
+            "single_kerr": pct(lat_ms[~kind]), "superposition": pct(lat_ms[kind]) if kind.any() else None,
+            "rays_per_s": rays / wall, "wall_s": wall, "multi_source_fraction": 0.3,
             "workload": "Poisson arrivals, lognormal rays (median 256^2, sigma 1, max 2048^2), "
                         "70% Kerr / 30% 2-16 hole superpositions"}
 

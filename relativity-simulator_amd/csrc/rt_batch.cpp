@@ -81,7 +81,7 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool til
         plan.result_bytes += results_bytes(P.n_rays, false);
     }
     const size_t nq = plan.q_params.size(), nm = plan.m_params.size();
-    plan.table_bytes = kCounterBytes + al256(nq * sizeof(TraceParams)) + al256(nq * sizeof(OutPtrs)) +
+    plan.table_bytes = kCounterBytes + al256(nm * 8) + al256(nq * sizeof(TraceParams)) + al256(nq * sizeof(OutPtrs)) +
                        al256((nq + 1) * sizeof(uint64_t)) + al256(nm * sizeof(MultiDev));
     return RT_OK;
 }
@@ -91,6 +91,8 @@ struct Placed {
     std::vector<OutPtrs> outs;           // per request (index = request index)
     QueueDev Q{};
     unsigned long long* counters = nullptr;
+    unsigned long long* heads = nullptr; // queue head of each superposition request
+    size_t n_zero = 0;                   // u64 words cleared before every launch (counters + heads)
     TraceParams* d_params = nullptr;
     OutPtrs* d_outs = nullptr;
     uint64_t* d_offset = nullptr;
@@ -102,13 +104,16 @@ void place_batch(const BatchPlan& plan, char* results, char* tables, const Launc
     pl.outs.assign(n, OutPtrs{});
     char* cur = results;
     char* tb = tables;
+    const size_t nq = plan.q_params.size(), nm = plan.m_params.size();
     pl.counters = reinterpret_cast<unsigned long long*>(tb);
     tb += kCounterBytes;
+    pl.heads = reinterpret_cast<unsigned long long*>(tb);
+    tb += al256(nm * 8);
+    pl.n_zero = (kCounterBytes + al256(nm * 8)) / 8;
     for (size_t i = 0; i < n; ++i) {
         pl.outs[i] = carve_results(cur, plan.n_rays[i], false);
         pl.outs[i].counters = pl.counters;
     }
-    const size_t nq = plan.q_params.size(), nm = plan.m_params.size();
     pl.d_params = reinterpret_cast<TraceParams*>(tb);
     tb += al256(nq * sizeof(TraceParams));
     pl.d_outs = reinterpret_cast<OutPtrs*>(tb);
@@ -124,7 +129,6 @@ void place_batch(const BatchPlan& plan, char* results, char* tables, const Launc
     pl.Q.total = plan.q_offset.back();
     pl.Q.n_req = static_cast<uint32_t>(nq);
     pl.Q.refill_min = std::max<uint32_t>(1, std::min<uint32_t>(shape.refill_min, 64));
-    (void)nm;
 }
 
 // Host->device copies of the batch tables (outside any captured graph).
@@ -150,14 +154,14 @@ rt_status upload_batch(const BatchPlan& plan, const Placed& pl, hipStream_t s) {
 // The launch sequence of a batch (captured into a graph by rt_trace_batch).
 hipError_t launch_batch(const BatchPlan& plan, const Placed& pl, const LaunchShape& shape, uint32_t grid,
                         hipStream_t s) {
-    hipError_t e = launch_zero_u64(pl.counters, kCounterBytes / 8, s);
+    hipError_t e = launch_zero_u64(pl.counters, static_cast<uint32_t>(pl.n_zero), s);
     if (e != hipSuccess) return e;
     if (!plan.q_params.empty()) {
         e = launch_trace_queue(pl.Q, shape, plan.mixed, grid, s);
         if (e != hipSuccess) return e;
     }
     for (size_t j = 0; j < plan.m_params.size(); ++j) {
-        e = launch_trace_multi(plan.m_params[j], pl.outs[plan.m_req[j]], pl.d_tables + j, s);
+        e = launch_trace_multi(plan.m_params[j], pl.outs[plan.m_req[j]], pl.d_tables + j, pl.heads + j, shape, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -361,7 +365,7 @@ struct rt_server {
     PoolAlloc alloc;
     std::vector<hipStream_t> streams;
     std::vector<bool> stream_busy;
-    LaunchShape qshape;
+    LaunchShape qshape = kQueueShape;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
@@ -537,7 +541,8 @@ extern "C" {
 rt_status rt_get_launch_shape(const rt_context* ctx, rt_launch_shape* plain, rt_launch_shape* queue) {
     if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
     if (plain) *plain = rt_launch_shape{ctx->shape.block, ctx->shape.waves, ctx->shape.refill_min, ctx->shape.lane_refill};
-    if (queue) *queue = rt_launch_shape{ctx->qshape.block, ctx->qshape.waves, ctx->qshape.refill_min, ctx->qshape.lane_refill};
+    if (queue) *queue = rt_launch_shape{ctx->qshape.block, ctx->qshape.waves, ctx->qshape.refill_min,
+                                        ctx->qshape.lane_refill};
     return RT_OK;
 }
 
@@ -569,13 +574,13 @@ rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample
     std::lock_guard<std::mutex> lk(ctx->mu);
     // candidates: tile queue at each block size and occupancy, lane-refill
     // queue at each block size with three refill thresholds
+    LaunchShape keep = ctx->qshape, win = ctx->qshape;
     std::vector<LaunchShape> cands;
     for (uint32_t b : {64u, 128u, 256u})
         for (uint32_t w : {1u, 2u}) {
             cands.push_back(LaunchShape{b, w, 16, 0});
             for (uint32_t r : {8u, 16u, 32u}) cands.push_back(LaunchShape{b, w, r, 1});
         }
-    LaunchShape keep = ctx->qshape, win = ctx->qshape;
     double best_ms = 1e300;
     for (const LaunchShape& c : cands) {
         ctx->qshape = c;

@@ -475,7 +475,7 @@ rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, 
     RT_HIP(launch_zero_u64(counters, 16, ctx->stream));
     RT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (multi)
-        RT_HIP(launch_trace_multi(P, O, Td, ctx->stream));
+        RT_HIP(launch_trace_multi(P, O, Td, counters + 8, ctx->qshape, ctx->stream));
     else if (cam)
         RT_HIP(launch_trace_camera(P, O, ctx->shape, ctx->stream));
     else
@@ -495,7 +495,7 @@ rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, 
     s.rejected_steps = cnt[1];
     s.rhs_evals = cnt[2];
     s.fp32_steps = cnt[3];
-    s.block_size = multi ? 256 : ctx->shape.block;
+    s.block_size = multi ? ctx->qshape.block : ctx->shape.block;
     s.grid_size = 0;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RT_OK;
@@ -572,7 +572,8 @@ rt_status rt_context_create(int device, rt_context** out) {
     rt_context* ctx = new (std::nothrow) rt_context();
     if (!ctx) return fail(RT_ERR_OOM, "context allocation");
     ctx->device = device;
-    ctx->qshape = ctx->shape;
+    ctx->shape = kPlainShape;
+    ctx->qshape = kQueueShape;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);

@@ -7,10 +7,11 @@
 namespace rt {
 
 // Launch shape chosen by the autotuner: threads per block (multiple of 64,
-// at most 256), the occupancy the kernel is compiled for (1 or 2 waves per
-// SIMD; 2 caps registers at 256 and spills in the once-per-ray event code)
-// and, for the work-queue kernel, how many lanes of a wave must be idle
-// before it refills them.
+// at most 256), the occupancy the kernel is compiled for (launch bounds of 1
+// or 2 waves per SIMD; both compile to 2 waves per SIMD for the plain FP64
+// kernels) and, for the work-queue kernels, the queue granularity and how
+// many lanes of a wave must be idle before the lane-refill kernel refills
+// them.
 struct LaunchShape {
     uint32_t block = 64;
     uint32_t waves = 2;
@@ -18,12 +19,21 @@ struct LaunchShape {
     uint32_t lane_refill = 0;  // work queue: 0 whole 8x8 tiles per wave, 1 per-lane refill
 };
 
+// Defaults, the fastest shapes measured on MI355X (4 x 2048^2 Kerr frames):
+// one-wave workgroups, whose wave slots the hardware refills as each 8x8 tile
+// finishes.
+constexpr LaunchShape kPlainShape{64, 1, 16, 0};
+constexpr LaunchShape kQueueShape{64, 2, 16, 0};
+
 // Camera frame (single Kerr): one lane per pixel, lanes of a wave on an 8x8 tile.
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
 // Explicit initial states (O.in_x / O.in_p), single Kerr.
 hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
 // Superposition (camera or explicit rays per P.mode), table T in device memory.
-hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, hipStream_t stream);
+// Lanes per ray of the superposition kernel (one source per lane).
+uint32_t multi_ray_lanes(uint32_t n_src);
+hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
+                              const LaunchShape& L, hipStream_t stream);
 // Persistent work-queue kernel over many single-Kerr requests (tiles or rays
 // per L.lane_refill; Q.req_offset counts tiles or rays accordingly).
 // grid 0: computed from the occupancy (not allowed inside a stream capture).

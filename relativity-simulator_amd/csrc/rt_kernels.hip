@@ -12,7 +12,9 @@
 //   k_trace_queue        persistent kernel over a device work queue of rays:
 //                        a wave refills its idle lanes with one atomic
 //                        (ballot + prefix count)
-//   k_trace_multi        superposition of up to 16 sources, table in LDS
+//   k_trace_multi        superposition of up to 16 sources: G lanes per ray
+//                        (one source each, sums by DPP butterflies),
+//                        persistent, idle groups refilled ray by ray
 #include <hip/hip_runtime.h>
 
 #include "rt_device.hpp"
@@ -558,35 +560,81 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
 }
 
 // ------------------------------------------------------- superposition
-// Right-hand side of the superposed Hamiltonian (ms_rhs,
-// software_compute_engine.hpp:2857-2870, with a carried p_t): flat part plus
-// the rotated KS terms of every source, sources read from LDS.
-__device__ __forceinline__ void rhs_multi(const MultiDev& T, const double y[8], double d[8]) {
-    d[0] = -y[4];
-    d[1] = y[5];
-    d[2] = y[6];
-    d[3] = y[7];
-    d[4] = 0.0;
-    d[5] = d[6] = d[7] = 0.0;
-    for (uint32_t i = 0; i < T.n; ++i) {
-        const SourceDev& s = T.src[i];
-        const double dx = y[1] - s.c[0], dy = y[2] - s.c[1], dz = y[3] - s.c[2];
-        const double lx = dx * s.fx[0] + dy * s.fx[1] + dz * s.fx[2];
-        const double ly = dx * s.fy[0] + dy * s.fy[1] + dz * s.fy[2];
-        const double lz = dx * s.ax[0] + dy * s.ax[1] + dz * s.ax[2];
-        const double px = y[5] * s.fx[0] + y[6] * s.fx[1] + y[7] * s.fx[2];
-        const double py = y[5] * s.fy[0] + y[6] * s.fy[1] + y[7] * s.fy[2];
-        const double pz = y[5] * s.ax[0] + y[6] * s.ax[1] + y[7] * s.ax[2];
-        double ths, c[3], q[3];
-        ks_terms<double>(lx, ly, lz, y[4], px, py, pz, s.m, s.a, s.a2, ths, c, q);
-        d[0] += ths;
-        d[1] += c[0] * s.fx[0] + c[1] * s.fy[0] + c[2] * s.ax[0];
-        d[2] += c[0] * s.fx[1] + c[1] * s.fy[1] + c[2] * s.ax[1];
-        d[3] += c[0] * s.fx[2] + c[1] * s.fy[2] + c[2] * s.ax[2];
-        d[5] += q[0] * s.fx[0] + q[1] * s.fy[0] + q[2] * s.ax[0];
-        d[6] += q[0] * s.fx[1] + q[1] * s.fy[1] + q[2] * s.ax[1];
-        d[7] += q[0] * s.fx[2] + q[1] * s.fy[2] + q[2] * s.ax[2];
+// Source-parallel layout. The G lanes of a group (G = 2, 4, 8 or 16, the
+// smallest power of two not below the number of sources) trace one ray
+// together: every lane holds the whole ray state, lane s of the group
+// evaluates source s, and sums over the sources are butterfly reductions
+// inside the group (DPP moves, no memory). Floating-point addition is
+// commutative bit for bit, so after each butterfly level both partners hold
+// identical bits: every lane of a group ends with the same sum and the lanes
+// take the same accept/reject and event decisions in lock step. A ray's
+// latency per step is then that of one source instead of all of them, which
+// matters because a superposition frame's time is set by its slowest rays
+// (rays that graze a horizon take thousands of steps).
+
+// DPP lane moves inside rows of 16: xor 1 and xor 2 (quad permutes), mirror
+// inside 8 lanes and inside 16 lanes. Each pairs every lane with a partner
+// in the same aligned group of 2, 4, 8 or 16 lanes.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror
+constexpr int kDppMirror = 0x140;     // row_mirror
+
+template <int Ctrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b)), Ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(b >> 32)), Ctrl, 0xF, 0xF,
+                                               false);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                          static_cast<uint32_t>(lo));
+}
+
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), Ctrl, 0xF, 0xF, false));
+}
+
+// Sum over the group (identical bits in every lane of the group).
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+    if constexpr (G == 1) return v;
+    v = v + dpp_f64<kDppXor1>(v);
+    if constexpr (G >= 4) v = v + dpp_f64<kDppXor2>(v);
+    if constexpr (G >= 8) v = v + dpp_f64<kDppHalfMirror>(v);
+    if constexpr (G >= 16) v = v + dpp_f64<kDppMirror>(v);
+    return v;
+}
+
+template <int G>
+__device__ __forceinline__ double group_min(double v) {
+    if constexpr (G == 1) return v;
+    v = fmin(v, dpp_f64<kDppXor1>(v));
+    if constexpr (G >= 4) v = fmin(v, dpp_f64<kDppXor2>(v));
+    if constexpr (G >= 8) v = fmin(v, dpp_f64<kDppHalfMirror>(v));
+    if constexpr (G >= 16) v = fmin(v, dpp_f64<kDppMirror>(v));
+    return v;
+}
+
+// Earliest event of the group: smallest s, ties to the lower source index
+// (the order in which a sequential scan over the sources would keep them).
+template <int Ctrl>
+__device__ __forceinline__ void argmin_step(double& s, uint32_t& i) {
+    const double so = dpp_f64<Ctrl>(s);
+    const uint32_t io = dpp_u32<Ctrl>(i);
+    if (so < s || (so == s && io < i)) {
+        s = so;
+        i = io;
     }
+}
+
+template <int G>
+__device__ __forceinline__ void group_argmin(double& s, uint32_t& i) {
+    if constexpr (G == 1) return;
+    argmin_step<kDppXor1>(s, i);
+    if constexpr (G >= 4) argmin_step<kDppXor2>(s, i);
+    if constexpr (G >= 8) argmin_step<kDppHalfMirror>(s, i);
+    if constexpr (G >= 16) argmin_step<kDppMirror>(s, i);
 }
 
 __device__ __forceinline__ void to_local(const SourceDev& s, double x, double y, double z, double l[3]) {
@@ -596,22 +644,107 @@ __device__ __forceinline__ void to_local(const SourceDev& s, double x, double y,
     l[2] = dx * s.ax[0] + dy * s.ax[1] + dz * s.ax[2];
 }
 
-// Superposed null constraint as a quadratic in p_t; projection as for one hole.
-__device__ __forceinline__ bool project_multi(const MultiDev& T, double y[8], double tol) {
-    double sh = 0.0, shq = 0.0, shq2 = 0.0;
-    for (uint32_t i = 0; i < T.n; ++i) {
-        const SourceDev& s = T.src[i];
-        double l[3];
-        to_local(s, y[1], y[2], y[3], l);
-        const double px = y[5] * s.fx[0] + y[6] * s.fx[1] + y[7] * s.fx[2];
-        const double py = y[5] * s.fy[0] + y[6] * s.fy[1] + y[7] * s.fy[2];
-        const double pz = y[5] * s.ax[0] + y[6] * s.ax[1] + y[7] * s.ax[2];
-        const Field f = ks_field(l[0], l[1], l[2], s.m, s.a, s.a2);
-        const double q = f.k0 * px + f.k1 * py + f.k2 * pz;
-        sh += f.H;
-        shq += f.H * q;
-        shq2 += f.H * q * q;
+// The source of one lane: source si of the table in LDS (has: si is below
+// the number of sources; G is never smaller than the number of sources).
+struct GroupSrc {
+    const SourceDev* S;
+    bool has;
+};
+
+// Right-hand side of the superposed Hamiltonian (ms_rhs,
+// software_compute_engine.hpp:2857-2870, with a carried p_t): flat part plus
+// the rotated KS terms of every source; each lane adds its sources and the
+// group sums.
+template <int G>
+__device__ __forceinline__ void rhs_group(const GroupSrc& gs, const double y[8], double d[8]) {
+    double ths = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0;
+    if (gs.has) {
+        const SourceDev* V = gs.S;
+        const double dx = y[1] - V->c[0], dy = y[2] - V->c[1], dz = y[3] - V->c[2];
+        double lx, ly, lz, px, py, pz;
+        {
+            const double f0 = V->fx[0], f1 = V->fx[1], f2 = V->fx[2];
+            lx = dx * f0 + dy * f1 + dz * f2;
+            px = y[5] * f0 + y[6] * f1 + y[7] * f2;
+        }
+        {
+            const double f0 = V->fy[0], f1 = V->fy[1], f2 = V->fy[2];
+            ly = dx * f0 + dy * f1 + dz * f2;
+            py = y[5] * f0 + y[6] * f1 + y[7] * f2;
+        }
+        {
+            const double f0 = V->ax[0], f1 = V->ax[1], f2 = V->ax[2];
+            lz = dx * f0 + dy * f1 + dz * f2;
+            pz = y[5] * f0 + y[6] * f1 + y[7] * f2;
+        }
+        double t, c[3], q[3];
+        ks_terms<double>(lx, ly, lz, y[4], px, py, pz, V->m, V->a, V->a2, t, c, q);
+        ths += t;
+        {
+            const double f0 = V->fx[0], f1 = V->fx[1], f2 = V->fx[2];
+            c0 += c[0] * f0;
+            c1 += c[0] * f1;
+            c2 += c[0] * f2;
+            q0 += q[0] * f0;
+            q1 += q[0] * f1;
+            q2 += q[0] * f2;
+        }
+        {
+            const double f0 = V->fy[0], f1 = V->fy[1], f2 = V->fy[2];
+            c0 += c[1] * f0;
+            c1 += c[1] * f1;
+            c2 += c[1] * f2;
+            q0 += q[1] * f0;
+            q1 += q[1] * f1;
+            q2 += q[1] * f2;
+        }
+        {
+            const double f0 = V->ax[0], f1 = V->ax[1], f2 = V->ax[2];
+            c0 += c[2] * f0;
+            c1 += c[2] * f1;
+            c2 += c[2] * f2;
+            q0 += q[2] * f0;
+            q1 += q[2] * f1;
+            q2 += q[2] * f2;
+        }
     }
+    d[0] = -y[4] + group_sum<G>(ths);
+    d[1] = y[5] + group_sum<G>(c0);
+    d[2] = y[6] + group_sum<G>(c1);
+    d[3] = y[7] + group_sum<G>(c2);
+    d[4] = 0.0;
+    d[5] = group_sum<G>(q0);
+    d[6] = group_sum<G>(q1);
+    d[7] = group_sum<G>(q2);
+}
+
+// Local coordinates of a world point in the frame of source V.
+__device__ __forceinline__ void to_local_v(const SourceDev* V, double x, double y, double z, double l[3]) {
+    const double dx = x - V->c[0], dy = y - V->c[1], dz = z - V->c[2];
+    l[0] = dx * V->fx[0] + dy * V->fx[1] + dz * V->fx[2];
+    l[1] = dx * V->fy[0] + dy * V->fy[1] + dz * V->fy[2];
+    l[2] = dx * V->ax[0] + dy * V->ax[1] + dz * V->ax[2];
+}
+
+// Superposed null constraint as a quadratic in p_t; projection as for one
+// hole. Counts the projection in L.
+template <int G>
+__device__ __forceinline__ bool project_group(const GroupSrc& gs, double y[8], double tol, Lane& L) {
+    double h = 0.0, hq = 0.0, hq2 = 0.0;
+    if (gs.has) {
+        const SourceDev* V = gs.S;
+        double l[3];
+        to_local_v(V, y[1], y[2], y[3], l);
+        const double px = y[5] * V->fx[0] + y[6] * V->fx[1] + y[7] * V->fx[2];
+        const double py = y[5] * V->fy[0] + y[6] * V->fy[1] + y[7] * V->fy[2];
+        const double pz = y[5] * V->ax[0] + y[6] * V->ax[1] + y[7] * V->ax[2];
+        const Field f = ks_field(l[0], l[1], l[2], V->m, V->a, V->a2);
+        const double q = f.k0 * px + f.k1 * py + f.k2 * pz;
+        h += f.H;
+        hq += f.H * q;
+        hq2 += f.H * q * q;
+    }
+    const double sh = group_sum<G>(h), shq = group_sum<G>(hq), shq2 = group_sum<G>(hq2);
     const double P2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
     const double A = -(1.0 + 2.0 * sh);
     const double B = 4.0 * shq;
@@ -625,6 +758,8 @@ __device__ __forceinline__ bool project_multi(const MultiDev& T, double y[8], do
     const double r1 = (-B + sd) / (2.0 * A);
     const double r2 = (-B - sd) / (2.0 * A);
     y[4] = (fabs(r1 - y[4]) <= fabs(r2 - y[4])) ? r1 : r2;
+    L.flags |= RT_FLAG_PROJECTED;
+    ++L.proj;
     return true;
 }
 
@@ -659,217 +794,263 @@ __device__ __forceinline__ void camera_state_multi(const TraceParams& P, const M
     y[7] = mom2 * inv_pt;
 }
 
-// Trace one ray through a superposition (FP64): DP45 as for one hole; events
-// per source (horizon of its local KS radius, crossing of its local equatorial
+// A ray through a superposition (FP64): DP45 as for one hole; events per
+// source (horizon of its local KS radius, crossing of its local equatorial
 // plane inside its disk), escape on the Euclidean radius while outgoing.
-__device__ void trace_multi(const TraceParams& P, const MultiDev& T, const double y0[8], Lane& L) {
+// multi_begin sets a group up, multi_step makes one attempt and returns true
+// (in every lane of the group) once the ray has terminated.
+template <int G>
+__device__ __forceinline__ void multi_begin(const TraceParams& P, const GroupSrc& gs, const double y0[8], Lane& L) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
-    rhs_multi(T, L.y, L.k1);
+    rhs_group<G>(gs, L.y, L.k1);
     L.rhs = 1;
     L.proj = L.f32 = 0;
     L.acc = L.rej = L.flags = L.cls = 0;
     L.lambda = 0.0;
     L.g = 0.0;
-    {
-        double scale = 1e300, blend = 1.0;
-        for (uint32_t i = 0; i < T.n; ++i) {
-            const SourceDev& s = T.src[i];
-            double l[3];
-            to_local(s, L.y[1], L.y[2], L.y[3], l);
-            const double r = ks_radius(l[0], l[1], l[2], s.a2);
-            scale = fmin(scale, sqrt(r * fmax(r - s.rh, 0.02 * s.m)));
-            blend = fmin(blend, fmin(fmax((r - 20.0 * s.rh) / (80.0 * s.rh), 0.0), 1.0));
-        }
-        const double ah = (P.h0 > 0.0) ? P.h0 : fmin(fmax(0.5 * scale * (1.0 + 6.8 * blend), 0.004), 3.5 * 7.8);
-        L.ah = fmin(fmax(ah, P.h_min), P.h_max);
+    double scale = 1e300, blend = 1.0;
+    if (gs.has) {
+        const SourceDev* V = gs.S;
+        double l[3];
+        to_local_v(V, L.y[1], L.y[2], L.y[3], l);
+        const double r = ks_radius(l[0], l[1], l[2], V->a2);
+        const double rh = V->rh;
+        scale = fmin(scale, sqrt(r * fmax(r - rh, 0.02 * V->m)));
+        blend = fmin(blend, fmin(fmax((r - 20.0 * rh) / (80.0 * rh), 0.0), 1.0));
     }
-    for (;;) {
-        if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
-            L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
-            bool near = false;
-            for (uint32_t i = 0; i < T.n; ++i) {
-                const SourceDev& s = T.src[i];
-                const double dx = L.y[1] - s.c[0], dy = L.y[2] - s.c[1], dz = L.y[3] - s.c[2];
-                near = near || (sqrt(dx * dx + dy * dy + dz * dz) < 6.0 * s.m);
-            }
-            L.cls = near ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
-            return;
-        }
-        const double ah = L.ah, h = -ah;
-        double k2[8], k3[8], k4[8], k5[8], k6[8], k7[8], Y[8], y5[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) Y[i] = L.y[i] + h * (dp::a21 * L.k1[i]);
-        rhs_multi(T, Y, k2);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) Y[i] = L.y[i] + h * (dp::a31 * L.k1[i] + dp::a32 * k2[i]);
-        rhs_multi(T, Y, k3);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) Y[i] = L.y[i] + h * (dp::a41 * L.k1[i] + dp::a42 * k2[i] + dp::a43 * k3[i]);
-        rhs_multi(T, Y, k4);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            Y[i] = L.y[i] + h * (dp::a51 * L.k1[i] + dp::a52 * k2[i] + dp::a53 * k3[i] + dp::a54 * k4[i]);
-        rhs_multi(T, Y, k5);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            Y[i] = L.y[i] + h * (dp::a61 * L.k1[i] + dp::a62 * k2[i] + dp::a63 * k3[i] + dp::a64 * k4[i] +
-                                 dp::a65 * k5[i]);
-        rhs_multi(T, Y, k6);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            y5[i] = L.y[i] + h * (dp::a71 * L.k1[i] + dp::a73 * k3[i] + dp::a74 * k4[i] + dp::a75 * k5[i] +
-                                  dp::a76 * k6[i]);
-        rhs_multi(T, y5, k7);
-        L.rhs += 6;
-        double err = 0.0;
-        bool finite = true;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            finite = finite && isfinite(y5[i]);
-            if (i == 4) continue;
-            const double e = ah * fabs(dp::e1 * L.k1[i] + dp::e3 * k3[i] + dp::e4 * k4[i] + dp::e5 * k5[i] +
-                                       dp::e6 * k6[i] + dp::e7 * k7[i]);
-            const double sc = P.rtol * fmax(fabs(L.y[i]), fabs(y5[i])) + P.atol;
-            err = fmax(err, e * rcp64(sc));
-        }
-        if (!finite || !isfinite(err)) {
-            L.flags |= RT_FLAG_NONFINITE;
-            L.cls = RT_CLS_CAPTURED;
-            return;
-        }
-        if (!(err <= 1.0)) {
-            ++L.rej;
-            L.ah = next_step(ah, err, false, P.h_min, P.h_max);
-            continue;
-        }
-        ++L.acc;
-        Dense q[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            q[i].D = y5[i] - L.y[i];
-            q[i].B = h * L.k1[i] - q[i].D;
-            q[i].C = q[i].D - h * k7[i] - q[i].B;
-            q[i].E = h * (dp::d1 * L.k1[i] + dp::d3 * k3[i] + dp::d4 * k4[i] + dp::d5 * k5[i] + dp::d6 * k6[i] +
-                          dp::d7 * k7[i]);
-        }
-        auto xyz_at = [&](double ss, double out[3]) {
-            out[0] = dense_at(q[1], L.y[1], ss);
-            out[1] = dense_at(q[2], L.y[2], ss);
-            out[2] = dense_at(q[3], L.y[3], ss);
-        };
-        double best = 2.0, best_g = 0.0;
-        uint32_t best_cls = 0;
-        for (uint32_t i = 0; i < T.n; ++i) {
-            const SourceDev& s = T.src[i];
-            double l0[3], l1[3];
-            to_local(s, L.y[1], L.y[2], L.y[3], l0);
-            to_local(s, y5[1], y5[2], y5[3], l1);
-            const double r1 = ks_radius(l1[0], l1[1], l1[2], s.a2);
-            if (r1 <= s.r_hz) {
-                const double r0 = ks_radius(l0[0], l0[1], l0[2], s.a2);
-                if (r0 > s.r_hz) {
-                    const double sh = illinois_root(
-                        [&](double ss) {
-                            double x[3], l[3];
-                            xyz_at(ss, x);
-                            to_local(s, x[0], x[1], x[2], l);
-                            return ks_radius(l[0], l[1], l[2], s.a2) - s.r_hz;
-                        },
-                        r0 - s.r_hz, r1 - s.r_hz);
-                    if (sh < best) {
-                        best = sh;
-                        best_cls = RT_CLS_CAPTURED;
-                    }
-                }
-            }
-            const double z0 = l0[2], z1 = l1[2];
-            if (s.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0))) {
-                const double sd = illinois_root(
-                    [&](double ss) {
-                        double x[3], l[3];
-                        xyz_at(ss, x);
-                        to_local(s, x[0], x[1], x[2], l);
-                        return l[2];
-                    },
-                    z0, z1);
-                double x[3], l[3];
-                xyz_at(sd, x);
-                to_local(s, x[0], x[1], x[2], l);
-                const double rc = ks_radius(l[0], l[1], l[2], s.a2);
-                if (rc >= s.disk_in && rc <= s.disk_out && rc > s.rh102 && sd < best) {
-                    const double pw0 = dense_at(q[5], L.y[5], sd), pw1 = dense_at(q[6], L.y[6], sd),
-                                 pw2 = dense_at(q[7], L.y[7], sd);
-                    const double pl0 = pw0 * s.fx[0] + pw1 * s.fx[1] + pw2 * s.fx[2];
-                    const double pl1 = pw0 * s.fy[0] + pw1 * s.fy[1] + pw2 * s.fy[2];
-                    const double pl2 = pw0 * s.ax[0] + pw1 * s.ax[1] + pw2 * s.ax[2];
-                    const double pm = sqrt(pl0 * pl0 + pl1 * pl1 + pl2 * pl2);
-                    const double lzv = (l[0] * pl1 - l[1] * pl0) / fmax(pm, 1e-30);
-                    best = sd;
-                    best_cls = RT_CLS_DISK;
-                    best_g = disk_redshift(s.m, s.a, rc, lzv, P.clamp_g != 0);
-                }
-            }
-        }
-        {
-            const double R1 = sqrt(y5[1] * y5[1] + y5[2] * y5[2] + y5[3] * y5[3]);
-            if (R1 >= P.r_esc && -(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0) {
-                const double R0 = sqrt(L.y[1] * L.y[1] + L.y[2] * L.y[2] + L.y[3] * L.y[3]);
-                double se = 1.0;
-                if (R0 < P.r_esc)
-                    se = illinois_root(
-                        [&](double ss) {
-                            double x[3];
-                            xyz_at(ss, x);
-                            return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) - P.r_esc;
-                        },
-                        R0 - P.r_esc, R1 - P.r_esc);
-                if (se < best) {
-                    best = se;
-                    best_cls = RT_CLS_ESCAPED;
-                }
-            }
-        }
-        if (best_cls != 0) {
-            if (best >= 1.0) {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
-            } else {
-#pragma unroll
-                for (int i = 0; i < 8; ++i) L.y[i] = dense_at(q[i], L.y[i], best);
-            }
-            if (project_multi(T, L.y, P.inv_tol)) {
-                L.flags |= RT_FLAG_PROJECTED;
-                ++L.proj;
-            }
-            L.lambda += best * ah;
-            L.cls = best_cls;
-            L.g = best_g;
-            return;
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
-        L.lambda += ah;
-        if (project_multi(T, L.y, P.inv_tol)) {
-            L.flags |= RT_FLAG_PROJECTED;
-            ++L.proj;
-            rhs_multi(T, L.y, L.k1);
-            ++L.rhs;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) L.k1[i] = k7[i];
-        }
-        L.ah = next_step(ah, err, true, P.h_min, P.h_max);
-    }
+    scale = group_min<G>(scale);
+    blend = group_min<G>(blend);
+    const double ah = (P.h0 > 0.0) ? P.h0 : fmin(fmax(0.5 * scale * (1.0 + 6.8 * blend), 0.004), 3.5 * 7.8);
+    L.ah = fmin(fmax(ah, P.h_min), P.h_max);
 }
 
-// Stage the superposition table into LDS (whole workgroup).
-__device__ __forceinline__ void stage_table(MultiDev& sm, const MultiDev* __restrict__ T) {
-    const uint32_t words = sizeof(MultiDev) / 8;
-    const double* src = reinterpret_cast<const double*>(T);
-    double* dst = reinterpret_cast<double*>(&sm);
-    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
-    __syncthreads();
+// The seven stages of one DP45 attempt on the group right-hand side (as
+// dp45_stages for one hole; WithDense adds the dense-output combination).
+template <bool WithDense, int G>
+__device__ __forceinline__ void dp45_stages_group(const GroupSrc& gs, const double y[8], const double k1[8], double h,
+                                                  double y5[8], double ep[8], double k7[8], double dn[8]) {
+    double k2[8], k3[8], k4[8], k5[8], k6[8], Y[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a21 * k1[i]);
+    rhs_group<G>(gs, Y, k2);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a31 * k1[i] + dp::a32 * k2[i]);
+    rhs_group<G>(gs, Y, k3);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a41 * k1[i] + dp::a42 * k2[i] + dp::a43 * k3[i]);
+    rhs_group<G>(gs, Y, k4);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        Y[i] = y[i] + h * (dp::a51 * k1[i] + dp::a52 * k2[i] + dp::a53 * k3[i] + dp::a54 * k4[i]);
+    rhs_group<G>(gs, Y, k5);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        Y[i] = y[i] + h * (dp::a61 * k1[i] + dp::a62 * k2[i] + dp::a63 * k3[i] + dp::a64 * k4[i] + dp::a65 * k5[i]);
+    rhs_group<G>(gs, Y, k6);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        y5[i] = y[i] + h * (dp::a71 * k1[i] + dp::a73 * k3[i] + dp::a74 * k4[i] + dp::a75 * k5[i] + dp::a76 * k6[i]);
+        ep[i] = dp::e1 * k1[i] + dp::e3 * k3[i] + dp::e4 * k4[i] + dp::e5 * k5[i] + dp::e6 * k6[i];
+        if (WithDense) dn[i] = dp::d1 * k1[i] + dp::d3 * k3[i] + dp::d4 * k4[i] + dp::d5 * k5[i] + dp::d6 * k6[i];
+    }
+    rhs_group<G>(gs, y5, k7);
+}
+
+// Event location inside an accepted step [y, y5] of size h: every lane the
+// horizons and disks of its own sources, then the group's earliest (ties to
+// the lower source index, as a scan over the sources in order keeps them),
+// then escape on the Euclidean radius while outgoing. Recomputes the step
+// with the dense-output combination. Fills the end state; returns the class
+// (0: no event), the same in every lane of the group.
+template <int G>
+__device__ __forceinline__ uint32_t locate_event_group(const TraceParams& P, const GroupSrc& gs, uint32_t si,
+                                                       uint32_t lane0, const double y[8], const double k1[8], double h, bool es,
+                                                       double yend[8], double* s_out, double* g_out) {
+    double y5[8], ep[8], k7[8], dn[8];
+    dp45_stages_group<true, G>(gs, y, k1, h, y5, ep, k7, dn);
+    auto coef = [&](int i) {
+        Dense q;
+        q.D = y5[i] - y[i];
+        q.B = h * k1[i] - q.D;
+        q.C = q.D - h * k7[i] - q.B;
+        q.E = h * (dn[i] + dp::d7 * k7[i]);
+        return q;
+    };
+    const Dense qx = coef(1), qy = coef(2), qz = coef(3);
+    double best = 2.0, best_g = 0.0;
+    uint32_t best_cls = 0, best_src = 0xFFFFFFFFu;
+    if (gs.has) {
+        const SourceDev* V = gs.S;
+        auto local_at = [&](double ss, double l[3]) {
+            to_local_v(V, dense_at(qx, y[1], ss), dense_at(qy, y[2], ss), dense_at(qz, y[3], ss), l);
+        };
+        double l0[3], l1[3];
+        to_local_v(V, y[1], y[2], y[3], l0);
+        to_local_v(V, y5[1], y5[2], y5[3], l1);
+        const double a2 = V->a2, r_hz = V->r_hz;
+        const double r0 = ks_radius(l0[0], l0[1], l0[2], a2);
+        const double r1 = ks_radius(l1[0], l1[1], l1[2], a2);
+        if (r1 <= r_hz && r0 > r_hz) {
+            const double sh = illinois_root(
+                [&](double ss) {
+                    double l[3];
+                    local_at(ss, l);
+                    return ks_radius(l[0], l[1], l[2], a2) - r_hz;
+                },
+                r0 - r_hz, r1 - r_hz);
+            if (sh < best) {
+                best = sh;
+                best_cls = RT_CLS_CAPTURED;
+                best_src = si;
+            }
+        }
+        const double z0 = l0[2], z1 = l1[2];
+        if (V->disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0))) {
+            const double sd = illinois_root(
+                [&](double ss) {
+                    double l[3];
+                    local_at(ss, l);
+                    return l[2];
+                },
+                z0, z1);
+            double l[3];
+            local_at(sd, l);
+            const double rc = ks_radius(l[0], l[1], l[2], a2);
+            if (rc >= V->disk_in && rc <= V->disk_out && rc > V->rh102 && sd < best) {
+                const Dense q5 = coef(5), q6 = coef(6), q7 = coef(7);
+                const double pw0 = dense_at(q5, y[5], sd), pw1 = dense_at(q6, y[6], sd), pw2 = dense_at(q7, y[7], sd);
+                const double pl0 = pw0 * V->fx[0] + pw1 * V->fx[1] + pw2 * V->fx[2];
+                const double pl1 = pw0 * V->fy[0] + pw1 * V->fy[1] + pw2 * V->fy[2];
+                const double pl2 = pw0 * V->ax[0] + pw1 * V->ax[1] + pw2 * V->ax[2];
+                const double pm = sqrt(pl0 * pl0 + pl1 * pl1 + pl2 * pl2);
+                const double lzv = (l[0] * pl1 - l[1] * pl0) / fmax(pm, 1e-30);
+                best = sd;
+                best_cls = RT_CLS_DISK;
+                best_src = si;
+                best_g = disk_redshift(V->m, V->a, rc, lzv, P.clamp_g != 0);
+            }
+        }
+    }
+    if (G > 1) {
+        uint32_t win = best_src;
+        group_argmin<G>(best, win);
+        if (best < 2.0) {
+            const int from = static_cast<int>(lane0 + (win & (G - 1)));
+            best_cls = static_cast<uint32_t>(__shfl(static_cast<int>(best_cls), from, 64));
+            best_g = __shfl(best_g, from, 64);
+        }
+    }
+    if (!(best < 2.0)) best_cls = 0;
+    if (es) {
+        const double R0 = sqrt(y[1] * y[1] + y[2] * y[2] + y[3] * y[3]);
+        const double R1 = sqrt(y5[1] * y5[1] + y5[2] * y5[2] + y5[3] * y5[3]);
+        double se = 1.0;
+        if (R0 < P.r_esc)
+            se = illinois_root(
+                [&](double ss) {
+                    const double x = dense_at(qx, y[1], ss), yy = dense_at(qy, y[2], ss), z = dense_at(qz, y[3], ss);
+                    return sqrt(x * x + yy * yy + z * z) - P.r_esc;
+                },
+                R0 - P.r_esc, R1 - P.r_esc);
+        if (se < best) {
+            best = se;
+            best_cls = RT_CLS_ESCAPED;
+        }
+    }
+    if (best_cls == 0) return 0;
+    if (best >= 1.0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) yend[i] = y5[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) yend[i] = dense_at(coef(i), y[i], best);
+    }
+    *s_out = best;
+    *g_out = best_g;
+    return best_cls;
+}
+
+template <int G>
+__device__ __forceinline__ bool multi_step(const TraceParams& P, const GroupSrc& gs, uint32_t si, uint32_t lane0,
+                                           Lane& L) {
+    if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
+        L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
+        double near = 0.0;
+        if (gs.has) {
+            const SourceDev* V = gs.S;
+            const double dx = L.y[1] - V->c[0], dy = L.y[2] - V->c[1], dz = L.y[3] - V->c[2];
+            if (sqrt(dx * dx + dy * dy + dz * dz) < 6.0 * V->m) near = 1.0;
+        }
+        L.cls = (group_min<G>(-near) < 0.0) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
+        return true;
+    }
+    const double ah = L.ah, h = -ah;
+    double y5[8], ep[8], k7[8];
+    dp45_stages_group<false, G>(gs, L.y, L.k1, h, y5, ep, k7, nullptr);
+    L.rhs += 6;
+    double err = 0.0;
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        finite = finite && isfinite(y5[i]);
+        if (i == 4) continue;
+        const double e = ah * fabs(ep[i] + dp::e7 * k7[i]);
+        const double sc = P.rtol * fmax(fabs(L.y[i]), fabs(y5[i])) + P.atol;
+        err = fmax(err, e * rcp64(sc));
+    }
+    if (!finite || !isfinite(err)) {
+        L.flags |= RT_FLAG_NONFINITE;
+        L.cls = RT_CLS_CAPTURED;
+        return true;
+    }
+    if (!(err <= 1.0)) {
+        ++L.rej;
+        L.ah = next_step(ah, err, false, P.h_min, P.h_max);
+        return false;
+    }
+    ++L.acc;
+    // any event in this step? (a horizon reached or a disk plane crossed by one
+    // of the lane's sources, escape)
+    double mine = 0.0;
+    if (gs.has) {
+        const SourceDev* V = gs.S;
+        double l0[3], l1[3];
+        to_local_v(V, L.y[1], L.y[2], L.y[3], l0);
+        to_local_v(V, y5[1], y5[2], y5[3], l1);
+        const bool hz = ks_radius(l1[0], l1[1], l1[2], V->a2) <= V->r_hz;
+        const bool dk = V->disk_on && ((l0[2] < 0.0 && l1[2] >= 0.0) || (l0[2] > 0.0 && l1[2] <= 0.0));
+        if (hz || dk) mine = 1.0;
+    }
+    const bool src_ev = group_min<G>(-mine) < 0.0;
+    const double R1 = sqrt(y5[1] * y5[1] + y5[2] * y5[2] + y5[3] * y5[3]);
+    const bool es = (R1 >= P.r_esc) && (-(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0);
+    if (src_ev || es) {
+        double s_ev = 1.0, g_ev = 0.0, ye[8];
+        const uint32_t c = locate_event_group<G>(P, gs, si, lane0, L.y, L.k1, h, es, ye, &s_ev, &g_ev);
+        if (c != 0) {
+            L.rhs += 6;
+            project_group<G>(gs, ye, P.inv_tol, L);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) L.y[i] = ye[i];
+            L.lambda += s_ev * ah;
+            L.cls = c;
+            L.g = g_ev;
+            return true;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
+    L.lambda += ah;
+    if (project_group<G>(gs, L.y, P.inv_tol, L)) {
+        rhs_group<G>(gs, L.y, L.k1);
+        ++L.rhs;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) L.k1[i] = k7[i];
+    }
+    L.ah = next_step(ah, err, true, P.h_min, P.h_max);
+    return false;
 }
 
 // Store with BL coordinates taken in the frame of source 0 (the central hole).
@@ -892,50 +1073,98 @@ __device__ __forceinline__ void store_multi(const OutPtrs& O, uint64_t idx, cons
     O.rejected[idx] = L.rej;
 }
 
-__global__ __launch_bounds__(256) void k_trace_multi(TraceParams P, OutPtrs O, const MultiDev* __restrict__ Tg) {
+// Superposition request: a persistent kernel, G lanes per ray (G = 2, 4, 8
+// or 16, lane s of a group takes source s). Idle groups are
+// refilled from a device counter (ballot over the group leaders, one
+// atomicAdd per refill, rank by popcount), so a group whose ray terminates
+// takes the next ray while the slow rays of the wave continue. Camera rays
+// are handed out in 8x8-tile order (ray g: tile g / 64, pixel g % 64). The
+// source table is staged into LDS. *next starts at zero.
+template <int G, int Waves>
+__global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPtrs O,
+                                                            const MultiDev* __restrict__ Tg,
+                                                            unsigned long long* __restrict__ next,
+                                                            uint32_t refill_min) {
     __shared__ MultiDev sm;
-    stage_table(sm, Tg);
-    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    Tally tally;
-    bool valid;
+    {
+        const uint32_t words = sizeof(MultiDev) / 8;
+        const double* src = reinterpret_cast<const double*>(Tg);
+        double* dst = reinterpret_cast<double*>(&sm);
+        for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
+        __syncthreads();
+    }
+    const MultiDev& T = sm;
+    const uint32_t lane_id = threadIdx.x & 63;
+    const uint32_t si = lane_id & (G - 1);
+    const uint32_t lane0 = lane_id & ~static_cast<uint32_t>(G - 1);
+    const uint64_t below = (lane0 == 0) ? 0ull : (~0ull >> (64 - lane0));
+    const GroupSrc gs{&sm.src[si < T.n ? si : 0], si < T.n};
+    const uint64_t total = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.width + 7) / 8) * 64
+                                         : static_cast<uint64_t>(P.n_rays);
+    Lane L;
+    bool active = false, drained = false;
     uint64_t idx = 0;
-    double y0[8];
-    if (P.mode == 0) {
-        const uint32_t tile = static_cast<uint32_t>(t >> 6), l = static_cast<uint32_t>(t & 63);
-        const uint32_t px = (tile % P.tiles_x) * 8 + (l & 7);
-        const uint32_t py = (tile / P.tiles_x) * 8 + (l >> 3);
-        valid = (px < P.width) && (py < P.n_rays / P.width);
-        if (valid) {
-            camera_state_multi(P, sm, px, P.row_first + py, y0);
-            idx = static_cast<uint64_t>(py) * P.width + px;
-        }
-    } else {
-        valid = t < P.n_rays;
-        if (valid) {
-            idx = t;
+    Tally tally;
+    for (;;) {
+        const uint64_t idle = __ballot(!active && si == 0);
+        const uint32_t n_idle = __popcll(idle);
+        if (!drained && n_idle != 0 && (n_idle * G >= refill_min || n_idle * G == 64)) {
+            const uint32_t first = __ffsll(static_cast<unsigned long long>(idle)) - 1;
+            unsigned long long base = 0;
+            if (lane_id == first) base = atomicAdd(next, static_cast<unsigned long long>(n_idle));
+            base = __shfl(base, first, 64);
+            if (base + n_idle >= total) drained = true;
+            if (!active) {
+                const uint64_t g = base + __popcll(idle & below);
+                if (g < total) {
+                    double y0[8];
+                    bool ok = true;
+                    if (P.mode == 0) {
+                        const uint32_t tile = static_cast<uint32_t>(g >> 6), l = static_cast<uint32_t>(g & 63);
+                        const uint32_t px = (tile % P.tiles_x) * 8 + (l & 7);
+                        const uint32_t py = (tile / P.tiles_x) * 8 + (l >> 3);
+                        ok = (px < P.width) && (py < P.n_rays / P.width);
+                        if (ok) {
+                            camera_state_multi(P, T, px, P.row_first + py, y0);
+                            idx = static_cast<uint64_t>(py) * P.width + px;
+                        }
+                    } else {
+                        idx = g;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                y0[c] = O.in_x[c * O.n + t];
-                y0[4 + c] = O.in_p[c * O.n + t];
+                        for (int c = 0; c < 4; ++c) {
+                            y0[c] = O.in_x[c * O.n + g];
+                            y0[4 + c] = O.in_p[c * O.n + g];
+                        }
+                    }
+                    if (ok && P.mode == 0 && T.observer_inside) {
+                        // observer inside a horizon: captured where it stands
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
+                        L.acc = L.rej = L.flags = L.proj = L.f32 = L.rhs = 0;
+                        L.lambda = 0.0;
+                        L.g = 0.0;
+                        L.cls = RT_CLS_CAPTURED;
+                        lane_finish(L);
+                        if (si == 0) store_multi(O, idx, L, T);
+                    } else if (ok) {
+                        multi_begin<G>(P, gs, y0, L);
+                        active = true;
+                    }
+                }
             }
         }
-    }
-    if (valid) {
-        Lane L;
-        if (sm.observer_inside && P.mode == 0) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
-            L.acc = L.rej = L.flags = L.proj = L.f32 = 0;
-            L.rhs = 0;
-            L.lambda = 0.0;
-            L.g = 0.0;
-            L.cls = RT_CLS_CAPTURED;
-        } else {
-            trace_multi(P, sm, y0, L);
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
         }
-        lane_finish(L);
-        store_multi(O, idx, L, sm);
-        tally.add(L);
+        if (active && multi_step<G>(P, gs, si, lane0, L)) {
+            lane_finish(L);
+            if (si == 0) {
+                store_multi(O, idx, L, T);
+                tally.add(L);
+            }
+            active = false;
+        }
     }
     tally.flush(O.counters);
 }
@@ -975,18 +1204,50 @@ hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const Launc
     return P.mixed ? launch_rays_t<true>(P, O, L, stream) : launch_rays_t<false>(P, O, L, stream);
 }
 
-hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, hipStream_t stream) {
-    uint64_t threads;
-    if (P.mode == 0) {
-        if (P.width == 0 || P.n_rays % P.width != 0) return hipErrorInvalidValue;
-        threads = static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.width + 7) / 8) * 64;
-    } else {
-        threads = P.n_rays;
-    }
-    const uint32_t block = 256;
-    const uint32_t grid = static_cast<uint32_t>((threads + block - 1) / block);
-    hipLaunchKernelGGL(k_trace_multi, dim3(grid), dim3(block), 0, stream, P, O, T);
+// Lanes per superposition ray: the smallest power of two not below the
+// number of sources (one source per lane).
+uint32_t multi_ray_lanes(uint32_t n_src) {
+    uint32_t g = 2;
+    while (g < n_src && g < 16) g <<= 1;
+    return g;
+}
+
+template <int G>
+static hipError_t launch_multi_g(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
+                                 const LaunchShape& L, hipStream_t stream) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const void* fn = (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_multi<G, 2>)
+                                    : reinterpret_cast<const void*>(&k_trace_multi<G, 1>);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, static_cast<int>(L.block), 0);
+    if (e != hipSuccess) return e;
+    // no more lanes than G per ray (a small request leaves the rest of the device to others)
+    const uint64_t rays = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.width + 7) / 8) * 64
+                                        : static_cast<uint64_t>(P.n_rays);
+    const uint64_t want = (rays * G + L.block - 1) / L.block;
+    const uint32_t grid = static_cast<uint32_t>(
+        std::min<uint64_t>(want, static_cast<uint64_t>(cus) * static_cast<uint64_t>(per_cu > 0 ? per_cu : 1)));
+    const uint32_t refill = std::max<uint32_t>(1, std::min<uint32_t>(L.refill_min, 64));
+    if (L.waves >= 2)
+        hipLaunchKernelGGL((k_trace_multi<G, 2>), dim3(grid), dim3(L.block), 0, stream, P, O, T, next, refill);
+    else
+        hipLaunchKernelGGL((k_trace_multi<G, 1>), dim3(grid), dim3(L.block), 0, stream, P, O, T, next, refill);
     return hipGetLastError();
+}
+
+hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
+                              const LaunchShape& L, hipStream_t stream) {
+    if (!shape_ok(L)) return hipErrorInvalidValue;
+    if (P.mode == 0 && (P.width == 0 || P.n_rays % P.width != 0)) return hipErrorInvalidValue;
+    if (P.n_src < 2 || P.n_src > static_cast<uint32_t>(kMaxSources)) return hipErrorInvalidValue;
+    switch (multi_ray_lanes(P.n_src)) {
+        case 2: return launch_multi_g<2>(P, O, T, next, L, stream);
+        case 4: return launch_multi_g<4>(P, O, T, next, L, stream);
+        case 8: return launch_multi_g<8>(P, O, T, next, L, stream);
+        case 16: return launch_multi_g<16>(P, O, T, next, L, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <bool Mixed>
