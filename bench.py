@@ -4,7 +4,10 @@ Workload (one step): four 2048x2048 Kerr frames, spins a in {0, 0.5, 0.9,
 0.998}, M = 1, observer at r = 1000 M, inclination 60 deg, vertical FOV 20 deg,
 DP45 rtol 1e-10 / atol 1e-12, capture at r_+ + 1e-6, escape at r = 1000 M
 outgoing, terminal disk z = 0 for r in [r_isco, 20 M]. Ray states are
-generated on the device and results stay in HBM (rt_trace_camera_resident).
+generated on the device and results stay in HBM: the four frames are one
+batch (rt_trace_batch, one persistent kernel, replayed from a hipGraph);
+--path plain traces them with one launch per frame (rt_trace_camera_resident).
+--workload uneven512 times the 512-request batched workload instead.
 
 Prints one JSON line. value = rays/s over the timed steps. roofline: the
 tracing kernel's algorithmic FP64 FLOP per launch (right-hand-side
@@ -210,8 +213,8 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=SIDE)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--path", choices=["batch", "plain"], default="batch",
-                    help="batch: the four frames in one persistent work-queue kernel (hipGraph replay); "
-                         "plain: one kernel per frame, one lane per pixel")
+                    help="batch: the four frames in one persistent work-queue kernel, slow tiles first (hipGraph "
+                         "replay); plain: one kernel per frame, one lane per pixel")
     ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")

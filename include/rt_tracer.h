@@ -221,9 +221,8 @@ typedef struct rt_launch_shape {
                              1 = idle lanes are refilled ray by ray           */
 } rt_launch_shape;
 
-/* Shapes used by the per-frame kernels (plain) and the work-queue kernels
- * (queue; the superposition kernel takes its block, waves and refill_min
- * from it too). */
+/* Shapes used by the single-Kerr per-frame kernels (plain) and work-queue
+ * kernels (queue). The superposition kernel has a fixed shape. */
 rt_status rt_get_launch_shape(const rt_context* ctx, rt_launch_shape* plain, rt_launch_shape* queue);
 rt_status rt_set_launch_shape(rt_context* ctx, const rt_launch_shape* plain, const rt_launch_shape* queue);
 /* Time the work-queue kernel over candidate shapes on the given sample

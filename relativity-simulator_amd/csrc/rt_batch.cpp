@@ -39,7 +39,7 @@ size_t al256(size_t v) { return (v + 255) & ~size_t(255); }
 struct BatchPlan {
     std::vector<uint32_t> q_req;         // request index of each queue entry
     std::vector<TraceParams> q_params;
-    std::vector<uint64_t> q_offset;      // n_q + 1, in rays (lane refill) or tiles
+    std::vector<uint64_t> q_offset;      // 2 n_q + 1 segment starts, in rays (lane refill) or tiles
     std::vector<uint32_t> m_req;         // request index of each superposition entry
     std::vector<TraceParams> m_params;
     std::vector<MultiDev> m_tables;
@@ -53,7 +53,7 @@ struct BatchPlan {
 rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool tiles) {
     plan = BatchPlan{};
     plan.n_rays.resize(n);
-    plan.q_offset.push_back(0);
+    std::vector<uint64_t> hot_len, cold_len;  // queue entries per request, hot and other tiles
     for (size_t i = 0; i < n; ++i) {
         const rt_request& r = reqs[i];
         std::string why;
@@ -74,15 +74,22 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool til
             plan.q_params.push_back(P);
             const uint64_t tiles_y = (P.n_rays / P.width + 7) / 8;
             const uint64_t n_tiles = static_cast<uint64_t>(P.tiles_x) * tiles_y;
-            plan.q_offset.push_back(plan.q_offset.back() + (tiles ? n_tiles : n_tiles * 64));
+            const uint64_t n_hot = static_cast<uint64_t>(P.hot_w) * P.hot_h;
+            const uint64_t unit = tiles ? 1 : 64;
+            hot_len.push_back(n_hot * unit);
+            cold_len.push_back((n_tiles - n_hot) * unit);
             plan.mixed = plan.mixed || P.mixed;
         }
         plan.n_rays[i] = P.n_rays;
         plan.result_bytes += results_bytes(P.n_rays, false);
     }
+    // queue segments: every request's hot tiles first, then the other tiles
+    plan.q_offset.assign(1, 0);
+    for (uint64_t v : hot_len) plan.q_offset.push_back(plan.q_offset.back() + v);
+    for (uint64_t v : cold_len) plan.q_offset.push_back(plan.q_offset.back() + v);
     const size_t nq = plan.q_params.size(), nm = plan.m_params.size();
     plan.table_bytes = kCounterBytes + al256(nm * 8) + al256(nq * sizeof(TraceParams)) + al256(nq * sizeof(OutPtrs)) +
-                       al256((nq + 1) * sizeof(uint64_t)) + al256(nm * sizeof(MultiDev));
+                       al256((2 * nq + 1) * sizeof(uint64_t)) + al256(nm * sizeof(MultiDev));
     return RT_OK;
 }
 
@@ -119,7 +126,7 @@ void place_batch(const BatchPlan& plan, char* results, char* tables, const Launc
     pl.d_outs = reinterpret_cast<OutPtrs*>(tb);
     tb += al256(nq * sizeof(OutPtrs));
     pl.d_offset = reinterpret_cast<uint64_t*>(tb);
-    tb += al256((nq + 1) * sizeof(uint64_t));
+    tb += al256((2 * nq + 1) * sizeof(uint64_t));
     pl.d_tables = reinterpret_cast<MultiDev*>(tb);
     pl.Q.reqs = pl.d_params;
     pl.Q.outs = pl.d_outs;
@@ -139,8 +146,8 @@ rt_status upload_batch(const BatchPlan& plan, const Placed& pl, hipStream_t s) {
         for (size_t j = 0; j < nq; ++j) qo[j] = pl.outs[plan.q_req[j]];
         RT_HIP(hipMemcpyAsync(pl.d_params, plan.q_params.data(), nq * sizeof(TraceParams), hipMemcpyHostToDevice, s));
         RT_HIP(hipMemcpyAsync(pl.d_outs, qo.data(), nq * sizeof(OutPtrs), hipMemcpyHostToDevice, s));
-        RT_HIP(hipMemcpyAsync(pl.d_offset, plan.q_offset.data(), (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                              s));
+        RT_HIP(hipMemcpyAsync(pl.d_offset, plan.q_offset.data(), (2 * nq + 1) * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, s));
         // the kernels read these tables; wait for the pageable copies to land
         RT_HIP(hipStreamSynchronize(s));
     }
@@ -161,7 +168,7 @@ hipError_t launch_batch(const BatchPlan& plan, const Placed& pl, const LaunchSha
         if (e != hipSuccess) return e;
     }
     for (size_t j = 0; j < plan.m_params.size(); ++j) {
-        e = launch_trace_multi(plan.m_params[j], pl.outs[plan.m_req[j]], pl.d_tables + j, pl.heads + j, shape, s);
+        e = launch_trace_multi(plan.m_params[j], pl.outs[plan.m_req[j]], pl.d_tables + j, pl.heads + j, kMultiShape, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -226,6 +226,40 @@ void fill_sources(const rt_metric* mt, const rt_termination* tm, MultiDev& T) {
 // frame of kerr_schild_initial_momentum (core/kerr_schild_null_integrator.hpp:149-177).
 // Superposition: flat spherical observer position and the frame of
 // ms_initial_momentum (render/software_compute_engine.hpp:2919-2956).
+// The rectangle of tiles around the hole's image: rays with impact
+// parameter up to 10 M (the shadow, at most ~7 M across for |a| < M, and the
+// rays that orbit near its edge), where the rays that take the most steps
+// are. Flat-space estimate from the pinhole mapping of camera_outward: the
+// hole lies along -n_obs; a direction with components (f, r, u) on
+// (forward, right, up) lands at u = r / (f tan_half), v = u / (f tan_half).
+static void hot_tiles(TraceParams& P, double r_obs) {
+    const double n_obs[3] = {P.sth * P.cph, P.sth * P.sph, P.cth};
+    double cf = 0.0, cr = 0.0, cu = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        cf -= P.fwd[i] * n_obs[i];
+        cr -= P.right[i] * n_obs[i];
+        cu -= P.up[i] * n_obs[i];
+    }
+    if (!(cf > 1e-6) || !(P.tan_half > 0.0)) return;  // hole behind the camera
+    const double W = P.width, H = P.height;
+    const double u = cr / (cf * P.tan_half), v = cu / (cf * P.tan_half);
+    const double px = (u / P.aspect + 1.0) * 0.5 * W - 0.5;
+    const double py = (1.0 - v) * 0.5 * H - 0.5 - static_cast<double>(P.row_first);
+    const double s = std::min(1.0, 10.0 * P.M / r_obs);
+    const double alpha = std::asin(s);
+    const double r_pix = (alpha >= 1.5) ? 1e9 : std::tan(alpha) / P.tan_half * 0.5 * H + 8.0;
+    const double tiles_y = static_cast<double>((P.n_rays / P.width + 7) / 8);
+    const double x0 = std::max(0.0, std::floor((px - r_pix) / 8.0));
+    const double x1 = std::min(static_cast<double>(P.tiles_x) - 1.0, std::floor((px + r_pix) / 8.0));
+    const double y0 = std::max(0.0, std::floor((py - r_pix) / 8.0));
+    const double y1 = std::min(tiles_y - 1.0, std::floor((py + r_pix) / 8.0));
+    if (!(x1 >= x0) || !(y1 >= y0)) return;  // image off the traced window
+    P.hot_x0 = static_cast<uint32_t>(x0);
+    P.hot_y0 = static_cast<uint32_t>(y0);
+    P.hot_w = static_cast<uint32_t>(x1 - x0 + 1.0);
+    P.hot_h = static_cast<uint32_t>(y1 - y0 + 1.0);
+}
+
 void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T) {
     const uint32_t first = std::min(c->row_offset, c->height);
     const uint32_t last = c->row_count ? std::min(first + c->row_count, c->height) : c->height;
@@ -249,6 +283,8 @@ void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T) {
     P.cth = std::cos(th);
     P.sph = std::sin(ph);
     P.cph = std::cos(ph);
+    P.hot_x0 = P.hot_y0 = P.hot_w = P.hot_h = 0;
+    if (P.n_src < 2) hot_tiles(P, r_obs);
     if (P.n_src >= 2 && T) {
         P.X0[0] = r_obs * P.sth * P.cph;
         P.X0[1] = r_obs * P.sth * P.sph;
@@ -475,7 +511,7 @@ rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, 
     RT_HIP(launch_zero_u64(counters, 16, ctx->stream));
     RT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (multi)
-        RT_HIP(launch_trace_multi(P, O, Td, counters + 8, ctx->qshape, ctx->stream));
+        RT_HIP(launch_trace_multi(P, O, Td, counters + 8, kMultiShape, ctx->stream));
     else if (cam)
         RT_HIP(launch_trace_camera(P, O, ctx->shape, ctx->stream));
     else
@@ -495,7 +531,7 @@ rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, 
     s.rejected_steps = cnt[1];
     s.rhs_evals = cnt[2];
     s.fp32_steps = cnt[3];
-    s.block_size = multi ? ctx->qshape.block : ctx->shape.block;
+    s.block_size = multi ? kMultiShape.block : ctx->shape.block;
     s.grid_size = 0;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RT_OK;

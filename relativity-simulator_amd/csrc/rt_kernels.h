@@ -19,11 +19,14 @@ struct LaunchShape {
     uint32_t lane_refill = 0;  // work queue: 0 whole 8x8 tiles per wave, 1 per-lane refill
 };
 
-// Defaults, the fastest shapes measured on MI355X (4 x 2048^2 Kerr frames):
-// one-wave workgroups, whose wave slots the hardware refills as each 8x8 tile
-// finishes.
+// Defaults, the fastest shapes measured on MI355X: one-wave workgroups (the
+// hardware refills a wave slot as soon as its 8x8 tile is done) for the
+// single-Kerr kernels (4 x 2048^2 Kerr frames); for the superposition kernel
+// (2..16 sources, 512^2) two-wave occupancy with a refill once the whole wave
+// is idle.
 constexpr LaunchShape kPlainShape{64, 1, 16, 0};
-constexpr LaunchShape kQueueShape{64, 2, 16, 0};
+constexpr LaunchShape kQueueShape{64, 1, 16, 0};
+constexpr LaunchShape kMultiShape{64, 2, 64, 0};
 
 // Camera frame (single Kerr): one lane per pixel, lanes of a wave on an 8x8 tile.
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);

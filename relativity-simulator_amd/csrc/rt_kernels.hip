@@ -350,15 +350,57 @@ struct Tally {
     }
 };
 
+// Tile t of a frame (0 <= t < tiles) in hot-first order: the hot rectangle
+// row by row, then the other tiles row by row.
+__device__ __forceinline__ void tile_xy(const TraceParams& P, uint32_t t, uint32_t& tx, uint32_t& ty) {
+    const uint32_t TX = P.tiles_x;
+    const uint32_t n_hot = P.hot_w * P.hot_h;
+    if (t < n_hot) {
+        tx = P.hot_x0 + t % P.hot_w;
+        ty = P.hot_y0 + t / P.hot_w;
+        return;
+    }
+    uint32_t k = t - n_hot;
+    const uint32_t above = P.hot_y0 * TX;
+    if (k >= above) {
+        k -= above;
+        const uint32_t per_row = TX - P.hot_w;
+        const uint32_t mid = P.hot_h * per_row;
+        if (k < mid) {
+            ty = P.hot_y0 + k / per_row;
+            tx = k % per_row;
+            if (tx >= P.hot_x0) tx += P.hot_w;
+            return;
+        }
+        k = (P.hot_y0 + P.hot_h) * TX + (k - mid);
+    }
+    tx = k % TX;
+    ty = k / TX;
+}
+
+// Find the queue segment holding queue index g (req_offset[s] <= g < req_offset[s + 1]).
+__device__ __forceinline__ uint32_t find_segment(const uint64_t* req_offset, uint32_t n_seg, uint64_t g) {
+    uint32_t lo = 0, hi = n_seg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (req_offset[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------- plain kernels
 // Camera frame, single Kerr. Thread t -> 8x8 tile t/64, lane t%64.
 template <bool Mixed, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P, OutPtrs O) {
     const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint32_t tile = static_cast<uint32_t>(t >> 6);
     const uint32_t lane = static_cast<uint32_t>(t & 63);
-    const uint32_t px = (tile % P.tiles_x) * 8 + (lane & 7);
-    const uint32_t py = (tile / P.tiles_x) * 8 + (lane >> 3);
+    uint32_t tx, ty;
+    tile_xy(P, static_cast<uint32_t>(t >> 6), tx, ty);
+    const uint32_t px = tx * 8 + (lane & 7);
+    const uint32_t py = ty * 8 + (lane >> 3);
     const uint32_t rows = P.n_rays / P.width;
     Tally tally;
     if (px < P.width && py < rows) {
@@ -407,18 +449,6 @@ __global__ __launch_bounds__(256, Waves) void k_trace_rays_kerr(TraceParams P, O
 // (or none is active), which bounds the atomics and the divergent set-up code.
 // Each lane carries its own request; the request's parameters are read from
 // the device table (cached), so lanes of different requests share a wave.
-__device__ __forceinline__ uint32_t find_request(const QueueDev& Q, uint64_t g) {
-    uint32_t lo = 0, hi = Q.n_req;  // req_offset[lo] <= g < req_offset[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (Q.req_offset[mid] <= g)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 template <bool Mixed, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
     const uint32_t lane_id = threadIdx.x & 63;
@@ -441,15 +471,19 @@ __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
             if (!active) {
                 const uint64_t g = base + __popcll(idle & lt_mask);
                 if (g < Q.total) {
-                    req = find_request(Q, g);
-                    ray = g - Q.req_offset[req];
+                    const uint32_t seg = find_segment(Q.req_offset, 2 * Q.n_req, g);
+                    const bool hot = seg < Q.n_req;
+                    req = hot ? seg : seg - Q.n_req;
+                    ray = g - Q.req_offset[seg];
                     const TraceParams& P = Q.reqs[req];
                     double y0[8], xi = 0.0;
                     bool ok = true;
                     if (P.mode == 0) {
-                        const uint32_t tile = static_cast<uint32_t>(ray >> 6), l = static_cast<uint32_t>(ray & 63);
-                        const uint32_t px = (tile % P.tiles_x) * 8 + (l & 7);
-                        const uint32_t py = (tile / P.tiles_x) * 8 + (l >> 3);
+                        const uint32_t l = static_cast<uint32_t>(ray & 63);
+                        uint32_t tx, ty;
+                        tile_xy(P, static_cast<uint32_t>(ray >> 6) + (hot ? 0u : P.hot_w * P.hot_h), tx, ty);
+                        const uint32_t px = tx * 8 + (l & 7);
+                        const uint32_t py = ty * 8 + (l >> 3);
                         ok = (px < P.width) && (py < P.n_rays / P.width);
                         if (ok) {
                             camera_state_kerr(P, px, P.row_first + py, y0, &xi);
@@ -514,20 +548,15 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
         if (lane == 0) t = atomicAdd(Q.next, 1ull);
         t = uniform_u64(__shfl(t, 0, 64));
         if (t >= Q.total) break;
-        // batches hold camera requests only (plan_batch); tile t of request req
-        uint32_t lo = 0, hi = Q.n_req;  // req_offset[lo] <= t < req_offset[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (req_offset[mid] <= t)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const uint32_t req = lo;
-        const uint64_t tile = t - req_offset[req];
+        // batches hold camera requests only (plan_batch): segment seg, tile t of it
+        const uint32_t seg = find_segment(req_offset, 2 * Q.n_req, t);
+        const bool hot = seg < Q.n_req;
+        const uint32_t req = hot ? seg : seg - Q.n_req;
         const TraceParams& P = reqs[req];
-        const uint32_t px = static_cast<uint32_t>(tile % P.tiles_x) * 8 + (lane & 7);
-        const uint32_t py = static_cast<uint32_t>(tile / P.tiles_x) * 8 + (lane >> 3);
+        uint32_t tx, ty;
+        tile_xy(P, static_cast<uint32_t>(t - req_offset[seg]) + (hot ? 0u : P.hot_w * P.hot_h), tx, ty);
+        const uint32_t px = tx * 8 + (lane & 7);
+        const uint32_t py = ty * 8 + (lane >> 3);
         uint32_t c_acc = 0, c_rej = 0, c_rhs = 0, c_f32 = 0, c_proj = 0;
         if (px < P.width && py < P.n_rays / P.width) {
             double y0[8], xi = 0.0;
@@ -1120,9 +1149,11 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
                     double y0[8];
                     bool ok = true;
                     if (P.mode == 0) {
-                        const uint32_t tile = static_cast<uint32_t>(g >> 6), l = static_cast<uint32_t>(g & 63);
-                        const uint32_t px = (tile % P.tiles_x) * 8 + (l & 7);
-                        const uint32_t py = (tile / P.tiles_x) * 8 + (l >> 3);
+                        const uint32_t l = static_cast<uint32_t>(g & 63);
+                        uint32_t tx, ty;
+                        tile_xy(P, static_cast<uint32_t>(g >> 6), tx, ty);
+                        const uint32_t px = tx * 8 + (l & 7);
+                        const uint32_t py = ty * 8 + (l >> 3);
                         ok = (px < P.width) && (py < P.n_rays / P.width);
                         if (ok) {
                             camera_state_multi(P, T, px, P.row_first + py, y0);

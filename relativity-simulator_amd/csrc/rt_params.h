@@ -28,6 +28,12 @@ struct TraceParams {
     uint32_t width, height, row_first, n_rays;
     uint32_t tiles_x;   // 8x8 pixel tiles per row of tiles
     uint32_t mode;      // 0 camera, 1 explicit rays
+    // Tile order: the rectangle of tiles around the hole's image (the
+    // captured and near-critical rays, which take up to ~100x the steps of
+    // the rest) is traced first, then the other tiles row by row, so that
+    // the slow rays overlap with the bulk of the frame. Tile units, relative
+    // to the traced row window; hot_w * hot_h == 0: plain row-major order.
+    uint32_t hot_x0, hot_y0, hot_w, hot_h;
     double aspect, tan_half;
     double fwd[3], right[3], up[3];
     double sth, cth, sph, cph;       // observer angles
@@ -76,15 +82,18 @@ struct OutPtrs {
     uint64_t n;                   // leading dimension of the SoA arrays
 };
 
-// Work queue of the persistent kernel: requests laid end to end in one ray
-// index space; req_offset[i] is the first queue index of request i.
+// Work queue of the persistent kernels: the hot tiles of every request
+// first, then the remaining tiles of every request. Segment s < n_req holds
+// request s's hot tiles, segment n_req + s its other tiles; req_offset[s] is
+// the first queue index of segment s (2 n_req + 1 entries; queue indices
+// count tiles, or rays for the lane-refill kernel).
 struct QueueDev {
     const TraceParams* reqs;
     const OutPtrs* outs;
-    const uint64_t* req_offset;   // n_req + 1 entries
+    const uint64_t* req_offset;   // 2 n_req + 1 entries
     unsigned long long* next;     // queue head (atomic)
     unsigned long long* counters; // summed statistics, as OutPtrs::counters
-    uint64_t total;               // queue length (req_offset[n_req])
+    uint64_t total;               // queue length (req_offset[2 n_req])
     uint32_t n_req;
     uint32_t refill_min;          // refill a wave when at least this many lanes are idle
 };
