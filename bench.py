@@ -45,6 +45,12 @@ R_OBS = 1000.0
 FLOP_RHS = 178
 FLOP_ATTEMPT = 432
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per ray of the tracing kernels on MI355X: rocprofv3 FETCH_SIZE +
+# WRITE_SIZE per dispatch, each scaled by what the counter reads for a plain
+# device-to-device copy of known size (x2.000 and x1.000), over the rays of
+# the dispatch (4 x 2048^2 Kerr frames). A camera ray reads nothing and
+# writes 116 B of results.
+HBM_BYTES_PER_RAY = {"batch": 116.55, "plain": 119.86}
 METRIC = "rays/s Kerr 2048\u00b2 mixed vs FP64 at \u22641e-9 rel err; p50/p99 latency @QPS"
 
 
@@ -313,7 +319,8 @@ def main() -> int:
                    "launches_per_step": launches // args.steps, "block": shape[0], "grid": shape[1],
                    "autotuned": tuned},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": HBM_BYTES_PER_RAY[args.path] * rays / launches},
     }
     ctx.close()
     if not args.no_serve:
