@@ -212,6 +212,25 @@ rt_status rt_trace_camera_resident(rt_context* ctx, const rt_camera* cam, const 
 /* Copy the device-resident results of the last resident trace to the host. */
 rt_status rt_fetch_results(rt_context* ctx, rt_results* out);
 
+/* The reference's per-pixel output record (GpuPixelOutput,
+ * render/gpu_types.hpp:1111-1121): same 32-byte layout and field meaning.
+ * r, g, b, a are the shaded colour, which this library does not compute
+ * (0, 0, 0, 1); the caller's shading fills them from the class and redshift. */
+typedef struct rt_pixel {
+    float r, g, b, a;
+    float redshift;          /* g as a float                                   */
+    float affine_parameter;  /* lambda as a float                              */
+    uint32_t status_flags;   /* PixelFlags bits: 1 captured, 2 escaped, 4 disk */
+    uint32_t iterations_used;/* accepted DP45 steps                            */
+} rt_pixel;
+
+/* Pack the device-resident results of the last resident trace into rt_pixel
+ * records on the device and copy those (32 B per ray instead of 116) to the
+ * caller's array of n_rays records. Replaces the readback loop of
+ * VulkanComputeExecutor::dispatch_and_readback (render/vulkan_compute_executor.hpp:1467-1600),
+ * which fills a std::vector<GpuPixelOutput>. */
+rt_status rt_fetch_pixels(rt_context* ctx, rt_pixel* out);
+
 /* ------------------------------------------------------- launch shapes */
 typedef struct rt_launch_shape {
     uint32_t block;       /* threads per workgroup: 64, 128, 192 or 256          */

@@ -629,6 +629,7 @@ rt_status rt_context_destroy(rt_context* ctx) {
     rt::forget_graphs(ctx);
     ctx->arena.release();
     ctx->aux.release();
+    ctx->pix.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -660,6 +661,21 @@ rt_status rt_fetch_results(rt_context* ctx, rt_results* out) {
     if (ctx->n_last == 0) return fail(RT_ERR_INVALID_ARG, "no resident results");
     RT_HIP(hipSetDevice(ctx->device));
     return copy_results_to_host(ctx->last, ctx->n_last, out, ctx->stream);
+}
+
+rt_status rt_fetch_pixels(rt_context* ctx, rt_pixel* out) {
+    static_assert(sizeof(rt_pixel) == 32, "rt_pixel is the 32-byte GpuPixelOutput record");
+    if (!ctx || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->n_last == 0) return fail(RT_ERR_INVALID_ARG, "no resident results");
+    RT_HIP(hipSetDevice(ctx->device));
+    const size_t bytes = ctx->n_last * sizeof(rt_pixel);
+    rt_status st = ctx->pix.ensure(bytes);
+    if (st != RT_OK) return st;
+    RT_HIP(launch_pack_pixels(ctx->last, ctx->n_last, ctx->pix.base, ctx->stream));
+    RT_HIP(hipMemcpyAsync(out, ctx->pix.base, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RT_HIP(hipStreamSynchronize(ctx->stream));
+    return RT_OK;
 }
 
 rt_status rt_trace_camera(rt_context* ctx, const rt_camera* cam, const rt_metric* metric, const rt_integrator* integ,

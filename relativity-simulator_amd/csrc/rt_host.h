@@ -65,6 +65,7 @@ struct rt_context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     rt::DeviceArena arena;       // results of the last trace
     rt::DeviceArena aux;         // superposition table, queue tables, counters
+    rt::DeviceArena pix;         // packed pixel records (rt_fetch_pixels)
     rt::OutPtrs last{};          // device results of the last resident trace
     size_t n_last = 0;
     rt_stats stats{};

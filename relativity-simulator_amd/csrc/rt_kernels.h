@@ -45,5 +45,7 @@ hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixe
 hipError_t queue_grid(const LaunchShape& L, bool mixed, uint32_t* grid);
 // Zero n 64-bit counters with a kernel (no memset node in captured graphs).
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
+// Pack n results into 32-byte pixel records (rt_pixel layout) at out.
+hipError_t launch_pack_pixels(const OutPtrs& O, uint64_t n, void* out, hipStream_t stream);
 
 }  // namespace rt

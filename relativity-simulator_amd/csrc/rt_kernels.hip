@@ -1335,6 +1335,25 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Results -> the reference's 32-byte pixel record: (0, 0, 0, 1) colour,
+// float redshift and affine parameter, PixelFlags bits, accepted steps. Two
+// 16-byte stores per ray.
+__global__ void k_pack_pixels(OutPtrs O, uint64_t n, uint4* __restrict__ out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = static_cast<float>(O.g[i]);
+    const float lam = static_cast<float>(O.lambda[i]);
+    out[2 * i] = make_uint4(0u, 0u, 0u, __float_as_uint(1.0f));
+    out[2 * i + 1] = make_uint4(__float_as_uint(g), __float_as_uint(lam), O.cls[i] & 0xFFu, O.steps[i]);
+}
+
+hipError_t launch_pack_pixels(const OutPtrs& O, uint64_t n, void* out, hipStream_t stream) {
+    const uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_pack_pixels, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream, O, n,
+                       static_cast<uint4*>(out));
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream) {
     hipLaunchKernelGGL(k_zero_u64, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
     return hipGetLastError();

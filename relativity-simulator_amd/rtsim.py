@@ -81,6 +81,18 @@ class rt_camera(C.Structure):
                 ("row_offset", C.c_uint32), ("row_count", C.c_uint32)]
 
 
+class rt_pixel(C.Structure):
+    """The reference's 32-byte per-pixel record (GpuPixelOutput)."""
+    _fields_ = [("r", C.c_float), ("g", C.c_float), ("b", C.c_float), ("a", C.c_float),
+                ("redshift", C.c_float), ("affine_parameter", C.c_float), ("status_flags", C.c_uint32),
+                ("iterations_used", C.c_uint32)]
+
+
+PIXEL_DTYPE = np.dtype([("r", np.float32), ("g", np.float32), ("b", np.float32), ("a", np.float32),
+                        ("redshift", np.float32), ("affine_parameter", np.float32), ("status_flags", np.uint32),
+                        ("iterations_used", np.uint32)])
+
+
 class rt_results(C.Structure):
     _fields_ = [("cls", C.POINTER(C.c_uint32)), ("x", C.POINTER(C.c_double)), ("p", C.POINTER(C.c_double)),
                 ("bl", C.POINTER(C.c_double)), ("lam", C.POINTER(C.c_double)), ("g", C.POINTER(C.c_double)),
@@ -224,6 +236,7 @@ EXPORTS = [
     "rt_default_integrator", "rt_default_termination", "rt_default_camera", "rt_kerr_metric",
     "rt_abi_version", "rt_version", "rt_last_error", "rt_context_create", "rt_context_destroy",
     "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
+    "rt_fetch_pixels",
     "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
     "rt_server_wait", "rt_server_take", "rt_server_cancel",
 ]
@@ -252,6 +265,7 @@ def load(path: Optional[str] = None) -> C.CDLL:
     lib.rt_trace_camera_resident.argtypes = [C.c_void_p, P(rt_camera), P(rt_metric), P(rt_integrator),
                                              P(rt_termination), P(C.c_size_t)]
     lib.rt_fetch_results.argtypes = [C.c_void_p, P(rt_results)]
+    lib.rt_fetch_pixels.argtypes = [C.c_void_p, C.c_void_p]
     lib.rt_trace_batch.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_results)]
     lib.rt_get_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
     lib.rt_set_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
@@ -320,6 +334,12 @@ class Context:
         res = Results(n)
         _check(self.lib, self.lib.rt_fetch_results(self.h, C.byref(res.struct)))
         return res
+
+    def fetch_pixels(self, n: int) -> np.ndarray:
+        """The last resident trace as n 32-byte pixel records (PIXEL_DTYPE)."""
+        out = np.zeros(n, PIXEL_DTYPE)
+        _check(self.lib, self.lib.rt_fetch_pixels(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def trace_rays(self, x: np.ndarray, p: np.ndarray, metric: rt_metric, integ: rt_integrator,
                    term: rt_termination) -> Results:

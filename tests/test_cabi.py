@@ -32,7 +32,7 @@ def test_every_declared_symbol_is_exported():
 
 
 STRUCTS = ["rt_source", "rt_metric", "rt_integrator", "rt_termination", "rt_camera", "rt_results", "rt_stats",
-           "rt_request", "rt_server_config", "rt_launch_shape"]
+           "rt_request", "rt_server_config", "rt_launch_shape", "rt_pixel"]
 
 
 def test_struct_layout_matches_ctypes():
@@ -40,7 +40,8 @@ def test_struct_layout_matches_ctypes():
     for s in STRUCTS:
         src += f'printf("{s} %zu\\n", sizeof({s}));\n'
     for s, f in [("rt_integrator", "max_steps"), ("rt_camera", "width"), ("rt_metric", "src"),
-                 ("rt_request", "termination"), ("rt_stats", "block_size"), ("rt_results", "rejected")]:
+                 ("rt_request", "termination"), ("rt_stats", "block_size"), ("rt_results", "rejected"),
+                 ("rt_pixel", "redshift"), ("rt_pixel", "iterations_used")]:
         src += f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));\n'
     src += "return 0;}\n"
     with tempfile.TemporaryDirectory() as d:
@@ -58,6 +59,11 @@ def test_struct_layout_matches_ctypes():
     assert int(got["rt_request.termination"]) == rtsim.rt_request.termination.offset
     assert int(got["rt_stats.block_size"]) == rtsim.rt_stats.block_size.offset
     assert int(got["rt_results.rejected"]) == rtsim.rt_results.rejected.offset
+    # the reference's GpuPixelOutput (render/gpu_types.hpp:1111-1121): 4 colour floats, redshift at 16,
+    # affine parameter, flags, iterations at 28, 32 bytes
+    assert int(got["rt_pixel"]) == 32 == rtsim.PIXEL_DTYPE.itemsize
+    assert int(got["rt_pixel.redshift"]) == 16 == rtsim.PIXEL_DTYPE.fields["redshift"][1]
+    assert int(got["rt_pixel.iterations_used"]) == 28 == rtsim.PIXEL_DTYPE.fields["iterations_used"][1]
 
 
 def test_defaults_match_python_mirror():

@@ -64,6 +64,21 @@ def test_invalid_requests_are_refused(ctx):
     compare(ctx.trace_camera(cam, met, ig, tm), O.trace_camera(cam, met, ig, tm), "after errors")
 
 
+def test_pixel_records_match_results(ctx):
+    """rt_fetch_pixels: the reference's 32-byte GpuPixelOutput records packed
+    on the device from the same resident results rt_fetch_results returns."""
+    cam = rtsim.default_camera(1000.0, math.radians(60.0), 0.0, math.radians(1.6), 37, 21)
+    met, ig, tm = rtsim.kerr_metric(1.0, 0.9), rtsim.default_integrator(), rtsim.default_termination()
+    n = ctx.trace_camera_resident(cam, met, ig, tm)
+    res, pix = ctx.fetch(n), ctx.fetch_pixels(n)
+    assert {1, 2, 4} <= set(res.klass.tolist())
+    assert np.array_equal(pix["status_flags"], res.cls & 0xFF)
+    assert np.array_equal(pix["iterations_used"], res.steps)
+    assert np.array_equal(pix["redshift"], res.g.astype(np.float32))
+    assert np.array_equal(pix["affine_parameter"], res.lam.astype(np.float32))
+    assert np.all(pix["a"] == 1.0) and not np.any(pix["r"]) and not np.any(pix["g"]) and not np.any(pix["b"])
+
+
 def test_one_request_batch(ctx):
     cam = rtsim.default_camera(25.0, math.radians(80.0), 0.0, math.radians(30.0), 3, 5)
     req = rtsim.make_request(cam, rtsim.kerr_metric(1.0, 0.3), rtsim.default_integrator(),
