@@ -62,6 +62,8 @@ typedef enum rt_status {
 #define RT_FLAG_NONFINITE (1u << 10) /* state became non-finite               */
 #define RT_FLAG_PROJECTED (1u << 11) /* null-constraint projection fired once */
 #define RT_FLAG_FP32_STEPS (1u << 12)/* mixed precision: some steps ran FP32  */
+#define RT_FLAG_CARTER (1u << 13)    /* captured by the Carter test (RT_TERM_CARTER_CAPTURE),
+                                        end state where the test fired         */
 #define RT_FLAG_INVALID (1u << 15)   /* input ray rejected (not traced)       */
 
 /* ------------------------------------------------------------ parameters */
@@ -108,6 +110,16 @@ typedef struct rt_integrator {
 } rt_integrator;
 
 #define RT_TERM_NO_G_CLAMP 1u /* do not clamp disk g to [0.05, 4]                */
+/* Single Kerr: end a ray as captured, without following it to r_+ + eps, as
+ * soon as it moves inward inside the disk's inner radius (anywhere when the
+ * disk is off), when Carter's radial potential R(r) of its constants (taken
+ * from the initial state) has no root outside r_+: the certain-absorption
+ * verdict of CarterPhaseClassifier (optics/carter_ray_classifier.hpp:116-164),
+ * with a margin that leaves rays near the critical curve to the integrator.
+ * Such rays move inward from there on, cannot meet the disk again, and are
+ * captured. Their end state is where the test fired (flag RT_FLAG_CARTER).
+ * Off by default. */
+#define RT_TERM_CARTER_CAPTURE 2u
 
 typedef struct rt_termination {
     double horizon_eps;   /* captured when r <= r_+ + horizon_eps (1e-6)           */

@@ -45,8 +45,13 @@ struct Spacetime {
     double disk_in = 6, disk_out = 20, disk_spin = 0;
     bool disk_on = true, clamp_g = true;
     double eps_h = 1e-6, r_esc = 1000;
+    bool carter = false;              // RT_TERM_CARTER_CAPTURE (single Kerr)
+    double carter_gate = 1e300;       // the test runs inside this radius
     Source s[RT_MAX_SOURCES];
 };
+
+// Margin of the Carter test, relative to r^4 (R(r) / r^4 is dimensionless).
+constexpr double kCarterMargin = 1e-6;
 
 struct Integ {
     double rtol, atol, h0, h_min, h_max, inv_tol;
@@ -118,6 +123,8 @@ static Spacetime make_spacetime(const rt_metric* mt, const rt_termination* tm) {
     const double isco = kerr_isco(st.M, st.a);
     st.disk_in = (tm->disk_inner > 0) ? tm->disk_inner : isco;
     st.disk_out = max((tm->disk_outer > 0) ? tm->disk_outer : 20.0 * st.M, st.disk_in * 1.05);
+    st.carter = (tm->flags & RT_TERM_CARTER_CAPTURE) != 0;
+    st.carter_gate = st.disk_on ? st.disk_in : 1e300;
     return st;
 }
 
@@ -324,6 +331,15 @@ RayOut trace_ray(const Spacetime& st, const Integ& ig, const double y_init[8], d
     uint32_t acc = 0, rej = 0, flags = 0;
     uint32_t cls = 0;
     bool done = false;
+    // RT_TERM_CARTER_CAPTURE: the test runs once, on the initial state (the
+    // constants hold along the ray and are best conditioned far from the hole)
+    double r_prev = st.multi ? 0.0 : ks_radius(y[1], y[2], y[3], st.a);
+    bool carter_no_turn = false;
+    if (st.carter && !st.multi) {
+        double xi_c, eta_c;
+        carter_no_turn = carter_constants(y, st.a, r_prev, &xi_c, &eta_c) &&
+                         carter_no_turning_point(st.M, st.a, st.rh, xi_c, eta_c, kCarterMargin) == 1;
+    }
     while (!done) {
         if (acc + rej >= ig.max_steps || ah < ig.h_min) {
             flags |= (acc + rej >= ig.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
@@ -496,6 +512,18 @@ RayOut trace_ray(const Spacetime& st, const Integ& ig, const double y_init[8], d
                 f(y, k[0]);
             } else {
                 std::memcpy(k[0], k[6], sizeof(k[0]));
+            }
+            if (carter_no_turn) {
+                // no turning point: captured once moving inward inside the gate
+                const double r1 = ks_radius(y[1], y[2], y[3], st.a);
+                if (r1 < r_prev && r1 < st.carter_gate) {
+                    cls = RT_CLS_CAPTURED;
+                    flags |= RT_FLAG_CARTER;
+                    std::memcpy(o.y, y, sizeof(y));
+                    done = true;
+                    break;
+                }
+                r_prev = r1;
             }
             ah = dp45_next_step(ah, err, true, ig.h_min, ig.h_max);
         } else {
@@ -846,6 +874,9 @@ void rto_ms_initial_momentum(const rt_metric* mt, const double x[3], const doubl
 void rto_ks_from_bl(double r, double th, double ph, double a, double out[3]) { ks_position_from_bl(r, th, ph, a, out); }
 void rto_bl_from_ks(const double x[3], double a, double out[3]) { bl_from_ks(x, a, out); }
 double rto_isco(double M, double a) { return kerr_isco(M, a); }
+int rto_carter_verdict(double M, double a, double xi, double eta, double margin) {
+    return carter_no_turning_point(M, a, horizon_radius(M, a), xi, eta, margin);
+}
 double rto_redshift(double M, double a, double r, double xi, int clamp) { return disk_redshift(M, a, r, xi, clamp != 0); }
 
 // One DP45 attempt on the geodesic system from y with step h; fills y5, the

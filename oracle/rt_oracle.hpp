@@ -217,6 +217,70 @@ inline void ks_initial_momentum(const double x[3], double M, double a, const dou
 }
 
 // ------------------------------------------------------------ disk
+// Carter's radial potential of a null ray with E = 1, xi = L_z/E, eta = Q/E^2:
+//   R(r) = r^4 + (a^2 - xi^2 - eta) r^2 + 2M (eta + (xi - a)^2) r - a^2 eta
+// (the polynomial CarterPhaseClassifier::classify builds,
+// optics/carter_ray_classifier.hpp:130-134). A ray moving inward is certainly
+// absorbed when R has no root outside r_+ (:149-153). Here: R(r_+) >= 0
+// always (Delta(r_+) = 0), R -> +inf, so R has no root outside r_+ exactly
+// when R > 0 at r_+ and at every critical point (R' = 0) outside r_+. The
+// critical points solve the depressed cubic c^3 + (P/2) c + Qc/4 = 0 (closed
+// form: Cardano, or the trigonometric form for three real roots). Unlike the
+// reference classifier, which leaves eta < 0 to integration, rays with a
+// negative Carter constant go through the same test. Returns 1
+// (no root, every value above margin * r^4), 0 (a root: some critical value
+// below -margin * c^4), -1 (undecided: within the margin, near the critical
+// curve).
+inline int carter_no_turning_point(double M, double a, double rp, double xi, double eta, double margin) {
+    const double e = eta;  // eta < 0 (vortical rays) uses the same polynomial
+    const double P = a * a - xi * xi - e;
+    const double Qc = 2.0 * M * (e + (xi - a) * (xi - a));
+    const double R0 = -a * a * e;
+    auto R = [&](double r) { return ((r * r + P) * r + Qc) * r + R0; };
+    bool undecided = !(R(rp) > margin * rp * rp * rp * rp);
+    const double p = 0.5 * P, q = 0.25 * Qc;
+    double c[3];
+    int nc;
+    const double disc = 0.25 * q * q + p * p * p / 27.0;
+    if (disc > 0.0) {
+        const double sd = std::sqrt(disc);
+        c[0] = std::cbrt(-0.5 * q + sd) + std::cbrt(-0.5 * q - sd);
+        nc = 1;
+    } else {
+        const double m = 2.0 * std::sqrt(-p / 3.0);
+        const double arg = std::clamp(3.0 * q / (p * m), -1.0, 1.0);
+        const double th = std::acos(arg) / 3.0;
+        for (int k = 0; k < 3; ++k) c[k] = m * std::cos(th - 2.0943951023931957 * k);
+        nc = 3;
+    }
+    for (int k = 0; k < nc; ++k) {
+        if (!(c[k] > rp)) continue;
+        const double c4 = c[k] * c[k] * c[k] * c[k];
+        const double v = R(c[k]);
+        if (v < -margin * c4) return 0;
+        if (!(v > margin * c4)) undecided = true;
+    }
+    return undecided ? -1 : 1;
+}
+
+// Constants of a KS null state y = (t, x, y, z, p_t, p_x, p_y, p_z) about a
+// Kerr hole of spin a: xi = L_z / E and eta = Q / E^2, with E = -p_t, L_z =
+// x p_y - y p_x and p_theta of the spheroidal angle (KS and BL share theta).
+// Returns false near the spin axis, where Q is ill-conditioned.
+inline bool carter_constants(const double y[8], double a, double r, double* xi, double* eta) {
+    const double E = -y[4];
+    const double cth = y[3] / r;
+    const double s2 = 1.0 - cth * cth;
+    if (!(s2 > 1e-12) || !(E > 0.0)) return false;
+    const double sth = std::sqrt(s2);
+    const double Lz = y[1] * y[6] - y[2] * y[5];
+    const double pth = (cth / sth) * (y[1] * y[5] + y[2] * y[6]) - r * sth * y[7];
+    const double Q = pth * pth + cth * cth * (Lz * Lz / s2 - a * a * E * E);
+    *xi = Lz / E;
+    *eta = Q / (E * E);
+    return true;
+}
+
 // Restates kerr_isco_radius, render/software_compute_engine.hpp:1052-1061
 // (Bardeen–Press–Teukolsky prograde ISCO).
 inline double kerr_isco(double m, double a_spin) {

@@ -44,7 +44,8 @@ struct BatchPlan {
     std::vector<TraceParams> m_params;
     std::vector<MultiDev> m_tables;
     std::vector<size_t> n_rays;          // per request
-    bool mixed = false;
+    bool mixed = false;   // some request uses the mixed precision policy
+    bool carter = false;  // some request asks for RT_TERM_CARTER_CAPTURE
     size_t result_bytes = 0, table_bytes = 0;
 };
 
@@ -79,6 +80,7 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool til
             hot_len.push_back(n_hot * unit);
             cold_len.push_back((n_tiles - n_hot) * unit);
             plan.mixed = plan.mixed || P.mixed;
+            plan.carter = plan.carter || P.carter;
         }
         plan.n_rays[i] = P.n_rays;
         plan.result_bytes += results_bytes(P.n_rays, false);
@@ -164,7 +166,7 @@ hipError_t launch_batch(const BatchPlan& plan, const Placed& pl, const LaunchSha
     hipError_t e = launch_zero_u64(pl.counters, static_cast<uint32_t>(pl.n_zero), s);
     if (e != hipSuccess) return e;
     if (!plan.q_params.empty()) {
-        e = launch_trace_queue(pl.Q, shape, plan.mixed, grid, s);
+        e = launch_trace_queue(pl.Q, shape, plan.mixed, plan.carter, grid, s);
         if (e != hipSuccess) return e;
     }
     for (size_t j = 0; j < plan.m_params.size(); ++j) {
@@ -183,8 +185,8 @@ std::vector<unsigned char> graph_key(const BatchPlan& plan, const Placed& pl, co
     };
     put(&pl.Q, sizeof(pl.Q));
     put(&shape, sizeof(shape));
-    const bool mixed = plan.mixed;
-    put(&mixed, sizeof(mixed));
+    const bool variant[2] = {plan.mixed, plan.carter};
+    put(variant, sizeof(variant));
     for (size_t j = 0; j < plan.m_params.size(); ++j) {
         put(&plan.m_params[j], sizeof(TraceParams));
         put(&pl.outs[plan.m_req[j]], sizeof(OutPtrs));
@@ -249,7 +251,7 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
     if (!gc.exec || gc.key != key) {
         gc.reset();
         uint32_t grid = 0;
-        RT_HIP(queue_grid(ctx->qshape, plan.mixed, &grid));
+        RT_HIP(queue_grid(ctx->qshape, plan.mixed, plan.carter, &grid));
         RT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         const hipError_t le = launch_batch(plan, pl, ctx->qshape, grid, ctx->stream);
         hipGraph_t g = nullptr;
@@ -289,7 +291,7 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
     s.fp32_steps = cnt[3];
     s.block_size = ctx->qshape.block;
     uint32_t grid = 0;
-    (void)queue_grid(ctx->qshape, plan.mixed, &grid);
+    (void)queue_grid(ctx->qshape, plan.mixed, plan.carter, &grid);
     s.grid_size = grid;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RT_OK;

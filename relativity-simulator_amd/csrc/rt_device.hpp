@@ -326,6 +326,64 @@ __device__ __forceinline__ void bl_from_ks(double x, double y, double z, double 
 }
 
 // Wave-wide sum of a 64-bit counter (wave64).
+// Carter capture test (RT_TERM_CARTER_CAPTURE). Constants of the KS null
+// state y about a hole of spin a at KS radius r: xi = L_z / E, eta = Q / E^2
+// (E = -p_t, L_z = x p_y - y p_x, p_theta of the spheroidal angle). Then
+// Carter's radial potential R(r) = r^4 + (a^2 - xi^2 - eta) r^2 +
+// 2M (eta + (xi - a)^2) r - a^2 eta has no root outside r_+ exactly when it
+// is positive at r_+ and at its critical points outside r_+ (R -> +inf, and
+// R(r_+) >= 0 because Delta(r_+) = 0); the critical points are the real roots
+// of c^3 + P/2 c + Qc/4 = 0. true: no root, every value above margin * r^4
+// (certain capture for a ray moving inward); false: a root, or within the
+// margin (near the critical curve), or near the spin axis.
+__device__ __forceinline__ bool carter_certain_capture(const double y[8], double r, double M, double a, double rp,
+                                                       double margin) {
+    const double E = -y[4];
+    const double cth = y[3] / r;
+    const double s2 = 1.0 - cth * cth;
+    if (!(s2 > 1e-12) || !(E > 0.0)) return false;
+    const double sth = sqrt(s2);
+    const double Lz = y[1] * y[6] - y[2] * y[5];
+    const double pth = (cth / sth) * (y[1] * y[5] + y[2] * y[6]) - r * sth * y[7];
+    const double Q = pth * pth + cth * cth * (Lz * Lz / s2 - a * a * E * E);
+    const double xi = Lz / E, e = Q / (E * E);  // e < 0 (vortical rays): same polynomial
+    const double P = a * a - xi * xi - e;
+    const double Qc = 2.0 * M * (e + (xi - a) * (xi - a));
+    const double R0 = -a * a * e;
+    auto R = [&](double x) { return ((x * x + P) * x + Qc) * x + R0; };
+    if (!(R(rp) > margin * rp * rp * rp * rp)) return false;
+    const double p = 0.5 * P, q = 0.25 * Qc;
+    const double disc = 0.25 * q * q + p * p * p / 27.0;
+    double c[3];
+    int nc;
+    if (disc > 0.0) {
+        const double sd = sqrt(disc);
+        c[0] = cbrt(-0.5 * q + sd) + cbrt(-0.5 * q - sd);
+        nc = 1;
+    } else {
+        const double m = 2.0 * sqrt(-p / 3.0);
+        const double arg = fmin(fmax(3.0 * q / (p * m), -1.0), 1.0);
+        const double th = acos(arg) / 3.0;
+        c[0] = m * cos(th);
+        c[1] = m * cos(th - 2.0943951023931957);
+        c[2] = m * cos(th - 4.1887902047863905);
+        nc = 3;
+    }
+    for (int k = 0; k < nc; ++k) {
+        if (!(c[k] > rp)) continue;
+        const double c4 = c[k] * c[k] * c[k] * c[k];
+        if (!(R(c[k]) > margin * c4)) return false;
+    }
+    return true;
+}
+
+// Margin of the Carter test, relative to r^4.
+constexpr double kCarterMargin = 1e-6;
+// Lane-internal flag bit (not exported): the Carter test, made once on the
+// ray's initial state (its constants are those of the whole ray, and they are
+// best conditioned far from the hole), found no turning point.
+constexpr uint32_t kCarterNoTurn = 1u << 30;
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;

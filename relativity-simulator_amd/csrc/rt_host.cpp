@@ -164,6 +164,8 @@ void fill_params(const rt_metric* mt, const rt_integrator* ig, const rt_terminat
     P.disk_out = std::max((tm->disk_outer > 0.0) ? tm->disk_outer : 20.0 * P.M, P.disk_in * 1.05);
     P.rh102 = P.rh * 1.02;
     P.near_r = 3.0 * (2.0 * P.M);
+    P.carter = (!multi && (tm->flags & RT_TERM_CARTER_CAPTURE)) ? 1u : 0u;
+    P.carter_gate = P.disk_on ? P.disk_in : 1e300;
     P.n_src = multi ? mt->n_sources : 0u;
     P.rtol = ig->rtol;
     P.atol = ig->atol;

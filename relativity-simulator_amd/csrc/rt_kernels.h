@@ -40,9 +40,10 @@ hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const Mult
 // Persistent work-queue kernel over many single-Kerr requests (tiles or rays
 // per L.lane_refill; Q.req_offset counts tiles or rays accordingly).
 // grid 0: computed from the occupancy (not allowed inside a stream capture).
-hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, uint32_t grid, hipStream_t stream);
+hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, bool carter, uint32_t grid,
+                              hipStream_t stream);
 // Grid that fills the device at the kernel's occupancy.
-hipError_t queue_grid(const LaunchShape& L, bool mixed, uint32_t* grid);
+hipError_t queue_grid(const LaunchShape& L, bool mixed, bool carter, uint32_t* grid);
 // Zero n 64-bit counters with a kernel (no memset node in captured graphs).
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
 // Pack n results into 32-byte pixel records (rt_pixel layout) at out.

@@ -201,6 +201,7 @@ __device__ __forceinline__ double next_step_fast(double ah, double err, bool acc
 }
 
 // Start a ray: first RHS (FSAL seed) and initial step.
+template <bool Carter>
 __device__ __forceinline__ void lane_begin(const TraceParams& P, const double y0[8], double xi, Lane& L) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
@@ -214,6 +215,9 @@ __device__ __forceinline__ void lane_begin(const TraceParams& P, const double y0
     L.acc = L.rej = L.flags = L.cls = 0;
     L.rhs = 1;
     L.proj = L.f32 = 0;
+    if constexpr (Carter) {
+        if (P.carter && carter_certain_capture(L.y, L.r0, P.M, P.a, P.rh, kCarterMargin)) L.flags = kCarterNoTurn;
+    }
 }
 
 // One DP45 attempt of a single-Kerr ray. Returns true when the ray ended:
@@ -296,6 +300,13 @@ __device__ __forceinline__ bool lane_step(const TraceParams& P, Lane& L) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) L.k1[i] = k7[i];
     }
+    // RT_TERM_CARTER_CAPTURE: no turning point, moving inward inside the gate
+    if ((L.flags & kCarterNoTurn) && r1 < L.r0 && r1 < P.carter_gate) {
+        L.r0 = r1;
+        L.cls = RT_CLS_CAPTURED;
+        L.flags |= RT_FLAG_CARTER;
+        return true;
+    }
     L.r0 = r1;
     L.ah = fast ? next_step_fast(ah, err, true, P.h_min, P.h_max) : next_step(ah, err, true, P.h_min, P.h_max);
     return false;
@@ -306,7 +317,7 @@ __device__ __forceinline__ void lane_finish(Lane& L) {
     const uint32_t c = L.cls & RT_CLS_MASK;
     if (c == RT_CLS_ESCAPED) L.g = 1.0 / (-L.y[4]);
     if (c == RT_CLS_CAPTURED) L.g = 0.0;
-    L.cls |= L.flags;
+    L.cls |= L.flags & ~kCarterNoTurn;
     if (L.f32) L.cls |= RT_FLAG_FP32_STEPS;
 }
 
@@ -393,7 +404,7 @@ __device__ __forceinline__ uint32_t find_segment(const uint64_t* req_offset, uin
 
 // ------------------------------------------------------- plain kernels
 // Camera frame, single Kerr. Thread t -> 8x8 tile t/64, lane t%64.
-template <bool Mixed, int Waves>
+template <bool Mixed, int Waves, bool Carter>
 __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P, OutPtrs O) {
     const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint32_t lane = static_cast<uint32_t>(t & 63);
@@ -407,7 +418,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P,
         double y0[8], xi;
         camera_state_kerr(P, px, P.row_first + py, y0, &xi);
         Lane L;
-        lane_begin(P, y0, xi, L);
+        lane_begin<Carter>(P, y0, xi, L);
         while (!lane_step<Mixed>(P, L)) {
         }
         lane_finish(L);
@@ -418,7 +429,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P,
 }
 
 // Explicit initial states, single Kerr.
-template <bool Mixed, int Waves>
+template <bool Mixed, int Waves, bool Carter>
 __global__ __launch_bounds__(256, Waves) void k_trace_rays_kerr(TraceParams P, OutPtrs O) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     Tally tally;
@@ -431,7 +442,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_rays_kerr(TraceParams P, O
         }
         const double xi = (y0[1] * y0[6] - y0[2] * y0[5]) / (-y0[4]);
         Lane L;
-        lane_begin(P, y0, xi, L);
+        lane_begin<Carter>(P, y0, xi, L);
         while (!lane_step<Mixed>(P, L)) {
         }
         lane_finish(L);
@@ -449,7 +460,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_rays_kerr(TraceParams P, O
 // (or none is active), which bounds the atomics and the divergent set-up code.
 // Each lane carries its own request; the request's parameters are read from
 // the device table (cached), so lanes of different requests share a wave.
-template <bool Mixed, int Waves>
+template <bool Mixed, int Waves, bool Carter>
 __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
     const uint32_t lane_id = threadIdx.x & 63;
     const uint64_t lt_mask = (lane_id == 0) ? 0ull : (~0ull >> (64 - lane_id));
@@ -499,7 +510,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
                         xi = (y0[1] * y0[6] - y0[2] * y0[5]) / (-y0[4]);
                     }
                     if (ok) {
-                        lane_begin(P, y0, xi, L);
+                        lane_begin<Carter>(P, y0, xi, L);
                         active = true;
                     }
                 }
@@ -536,7 +547,7 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
 // The request tables come in as separate read-only, non-aliased arguments so
 // that their wave-uniform loads stay scalar loads the compiler may repeat
 // instead of holding the parameters in vector registers.
-template <bool Mixed, int Waves>
+template <bool Mixed, int Waves, bool Carter>
 __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const TraceParams* __restrict__ reqs,
                                                             const OutPtrs* __restrict__ outs,
                                                             const uint64_t* __restrict__ req_offset) {
@@ -562,7 +573,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
             double y0[8], xi = 0.0;
             camera_state_kerr(P, px, P.row_first + py, y0, &xi);
             Lane L;
-            lane_begin(P, y0, xi, L);
+            lane_begin<Carter>(P, y0, xi, L);
             while (!lane_step<Mixed>(P, L)) {
             }
             lane_finish(L);
@@ -1203,36 +1214,40 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
 // ------------------------------------------------------------ launchers
 static bool shape_ok(const LaunchShape& L) { return L.block != 0 && L.block <= 256 && (L.block & 63) == 0; }
 
-template <bool Mixed>
+template <bool Mixed, bool Carter>
 static hipError_t launch_camera_t(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
     const uint64_t tiles_y = (P.n_rays / P.width + 7) / 8;
     const uint64_t threads = static_cast<uint64_t>(P.tiles_x) * tiles_y * 64;
     const uint32_t grid = static_cast<uint32_t>((threads + L.block - 1) / L.block);
     if (L.waves >= 2)
-        hipLaunchKernelGGL((k_trace_camera_kerr<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, P, O);
+        hipLaunchKernelGGL((k_trace_camera_kerr<Mixed, 2, Carter>), dim3(grid), dim3(L.block), 0, stream, P, O);
     else
-        hipLaunchKernelGGL((k_trace_camera_kerr<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, P, O);
+        hipLaunchKernelGGL((k_trace_camera_kerr<Mixed, 1, Carter>), dim3(grid), dim3(L.block), 0, stream, P, O);
     return hipGetLastError();
 }
 
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
     if (!shape_ok(L) || P.width == 0 || P.n_rays % P.width != 0) return hipErrorInvalidValue;
-    return P.mixed ? launch_camera_t<true>(P, O, L, stream) : launch_camera_t<false>(P, O, L, stream);
+    if (P.carter)
+        return P.mixed ? launch_camera_t<true, true>(P, O, L, stream) : launch_camera_t<false, true>(P, O, L, stream);
+    return P.mixed ? launch_camera_t<true, false>(P, O, L, stream) : launch_camera_t<false, false>(P, O, L, stream);
 }
 
-template <bool Mixed>
+template <bool Mixed, bool Carter>
 static hipError_t launch_rays_t(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
     const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(P.n_rays) + L.block - 1) / L.block);
     if (L.waves >= 2)
-        hipLaunchKernelGGL((k_trace_rays_kerr<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, P, O);
+        hipLaunchKernelGGL((k_trace_rays_kerr<Mixed, 2, Carter>), dim3(grid), dim3(L.block), 0, stream, P, O);
     else
-        hipLaunchKernelGGL((k_trace_rays_kerr<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, P, O);
+        hipLaunchKernelGGL((k_trace_rays_kerr<Mixed, 1, Carter>), dim3(grid), dim3(L.block), 0, stream, P, O);
     return hipGetLastError();
 }
 
 hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
     if (!shape_ok(L)) return hipErrorInvalidValue;
-    return P.mixed ? launch_rays_t<true>(P, O, L, stream) : launch_rays_t<false>(P, O, L, stream);
+    if (P.carter)
+        return P.mixed ? launch_rays_t<true, true>(P, O, L, stream) : launch_rays_t<false, true>(P, O, L, stream);
+    return P.mixed ? launch_rays_t<true, false>(P, O, L, stream) : launch_rays_t<false, false>(P, O, L, stream);
 }
 
 // Lanes per superposition ray: the smallest power of two not below the
@@ -1281,52 +1296,56 @@ hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const Mult
     }
 }
 
-template <bool Mixed>
+template <bool Mixed, bool Carter>
 static const void* queue_fn(const LaunchShape& L) {
     if (L.lane_refill)
-        return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<Mixed, 2>)
-                              : reinterpret_cast<const void*>(&k_trace_queue<Mixed, 1>);
-    return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_tiles<Mixed, 2>)
-                          : reinterpret_cast<const void*>(&k_trace_tiles<Mixed, 1>);
+        return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<Mixed, 2, Carter>)
+                              : reinterpret_cast<const void*>(&k_trace_queue<Mixed, 1, Carter>);
+    return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_tiles<Mixed, 2, Carter>)
+                          : reinterpret_cast<const void*>(&k_trace_tiles<Mixed, 1, Carter>);
 }
 
-template <bool Mixed>
+template <bool Mixed, bool Carter>
 static hipError_t launch_queue_t(const QueueDev& Q, const LaunchShape& L, uint32_t grid, hipStream_t stream) {
     if (L.lane_refill) {
         if (L.waves >= 2)
-            hipLaunchKernelGGL((k_trace_queue<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, Q);
+            hipLaunchKernelGGL((k_trace_queue<Mixed, 2, Carter>), dim3(grid), dim3(L.block), 0, stream, Q);
         else
-            hipLaunchKernelGGL((k_trace_queue<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, Q);
+            hipLaunchKernelGGL((k_trace_queue<Mixed, 1, Carter>), dim3(grid), dim3(L.block), 0, stream, Q);
     } else {
         if (L.waves >= 2)
-            hipLaunchKernelGGL((k_trace_tiles<Mixed, 2>), dim3(grid), dim3(L.block), 0, stream, Q, Q.reqs, Q.outs,
-                               Q.req_offset);
+            hipLaunchKernelGGL((k_trace_tiles<Mixed, 2, Carter>), dim3(grid), dim3(L.block), 0, stream, Q, Q.reqs,
+                               Q.outs, Q.req_offset);
         else
-            hipLaunchKernelGGL((k_trace_tiles<Mixed, 1>), dim3(grid), dim3(L.block), 0, stream, Q, Q.reqs, Q.outs,
-                               Q.req_offset);
+            hipLaunchKernelGGL((k_trace_tiles<Mixed, 1, Carter>), dim3(grid), dim3(L.block), 0, stream, Q, Q.reqs,
+                               Q.outs, Q.req_offset);
     }
     return hipGetLastError();
 }
 
-hipError_t queue_grid(const LaunchShape& L, bool mixed, uint32_t* grid) {
+hipError_t queue_grid(const LaunchShape& L, bool mixed, bool carter, uint32_t* grid) {
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    const void* fn = mixed ? queue_fn<true>(L) : queue_fn<false>(L);
+    const void* fn = carter ? (mixed ? queue_fn<true, true>(L) : queue_fn<false, true>(L))
+                            : (mixed ? queue_fn<true, false>(L) : queue_fn<false, false>(L));
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, static_cast<int>(L.block), 0);
     if (e != hipSuccess) return e;
     *grid = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu > 0 ? per_cu : 1);
     return hipSuccess;
 }
 
-hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, uint32_t grid, hipStream_t stream) {
+hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, bool carter, uint32_t grid,
+                              hipStream_t stream) {
     if (!shape_ok(L) || Q.n_req == 0) return hipErrorInvalidValue;
     if (grid == 0) {
-        const hipError_t e = queue_grid(L, mixed, &grid);
+        const hipError_t e = queue_grid(L, mixed, carter, &grid);
         if (e != hipSuccess) return e;
     }
-    return mixed ? launch_queue_t<true>(Q, L, grid, stream) : launch_queue_t<false>(Q, L, grid, stream);
+    if (carter)
+        return mixed ? launch_queue_t<true, true>(Q, L, grid, stream) : launch_queue_t<false, true>(Q, L, grid, stream);
+    return mixed ? launch_queue_t<true, false>(Q, L, grid, stream) : launch_queue_t<false, false>(Q, L, grid, stream);
 }
 
 // Zero a small counter array with a kernel (safe inside a captured graph).

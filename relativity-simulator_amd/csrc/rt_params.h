@@ -16,6 +16,8 @@ struct TraceParams {
     double r_esc;       // escape radius (KS r, or |x| for superpositions)
     double disk_in, disk_out, disk_spin, rh102;
     double near_r;      // 3 r_s: budget/step-underflow capture radius
+    double carter_gate; // Carter capture test runs inside this radius (disk_in, or everywhere)
+    uint32_t carter;    // 1: RT_TERM_CARTER_CAPTURE (single Kerr)
     uint32_t disk_on, clamp_g;
     uint32_t n_src;     // 0: single Kerr; >= 2: superposition (MultiDev)
     uint32_t mixed;     // 1: mixed precision policy

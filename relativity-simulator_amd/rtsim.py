@@ -36,10 +36,12 @@ RT_FLAG_MIN_STEP = 1 << 9
 RT_FLAG_NONFINITE = 1 << 10
 RT_FLAG_PROJECTED = 1 << 11
 RT_FLAG_FP32_STEPS = 1 << 12
+RT_FLAG_CARTER = 1 << 13
 RT_FLAG_INVALID = 1 << 15
 RT_PRECISION_FP64 = 0
 RT_PRECISION_MIXED = 1
 RT_TERM_NO_G_CLAMP = 1
+RT_TERM_CARTER_CAPTURE = 2
 
 STATUS_NAMES = {
     0: "RT_OK", 1: "RT_ERR_INVALID_ARG", 2: "RT_ERR_OOM", 3: "RT_ERR_DEVICE", 4: "RT_ERR_TIMEOUT",

@@ -61,6 +61,8 @@ def oracle():
         lib.rto_bl_from_ks.argtypes = [PD, D, PD]
         lib.rto_isco.argtypes = [D, D]
         lib.rto_isco.restype = D
+        lib.rto_carter_verdict.argtypes = [D, D, D, D, D]
+        lib.rto_carter_verdict.restype = C.c_int
         lib.rto_redshift.argtypes = [D, D, D, D, C.c_int]
         lib.rto_redshift.restype = D
         lib.rto_dp45_attempt.argtypes = [PD, D, P(rt_metric), D, D, PD, PD]

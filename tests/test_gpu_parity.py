@@ -94,3 +94,38 @@ def test_row_window(ctx):
     assert gpu.n == 7 * 24
     ref = O.trace_camera(cam, met, ig, tm)
     compare(gpu, ref, "rows")
+
+
+def subset(res, mask):
+    out = rtsim.Results(int(mask.sum()))
+    out.cls[:], out.lam[:], out.g[:] = res.cls[mask], res.lam[mask], res.g[mask]
+    out.steps[:], out.rejected[:] = res.steps[mask], res.rejected[mask]
+    out.x[:], out.p[:], out.bl[:] = res.x[:, mask], res.p[:, mask], res.bl[:, mask]
+    return out
+
+
+@pytest.mark.parametrize("spin,disk", [(0.998, True), (0.5, False)])
+def test_carter_capture_vs_oracle(ctx, spin, disk):
+    """RT_TERM_CARTER_CAPTURE: the same rays end early (RT_FLAG_CARTER) on the
+    GPU and in the oracle, at the same step; every other ray meets the usual
+    bar. The early rays are captured rays: they end inside the disk's inner
+    edge, close to the horizon, where the KS azimuth winds and rounding
+    differences grow, so (as for captured rays that end on the horizon) their
+    class, steps, r, theta and lambda are compared, to 1e-7 relative, and
+    their KS position to 1e-6."""
+    cam = rtsim.default_camera(1000.0, math.radians(60.0), 0.0, math.radians(1.6), 40, 40)
+    met = rtsim.kerr_metric(1.0, spin)
+    ig = rtsim.default_integrator()
+    tm = rtsim.default_termination(disk_enabled=disk, flags=rtsim.RT_TERM_CARTER_CAPTURE)
+    gpu = ctx.trace_camera(cam, met, ig, tm)
+    ref = O.trace_camera(cam, met, ig, tm)
+    early = (ref.cls & rtsim.RT_FLAG_CARTER) != 0
+    assert early.sum() > 10
+    assert np.array_equal(gpu.cls, ref.cls)
+    compare(subset(gpu, ~early), subset(ref, ~early), f"carter a={spin} disk={disk}, other rays")
+    assert np.array_equal(gpu.steps[early], ref.steps[early])
+    errs = {"x": rel(gpu.x[1:, early], ref.x[1:, early]).max(), "r": rel(gpu.bl[0, early], ref.bl[0, early]).max(),
+            "theta": rel(gpu.bl[1, early], ref.bl[1, early]).max(), "lambda": rel(gpu.lam[early], ref.lam[early]).max()}
+    print(f"[carter early {early.sum()} rays] " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v < (1e-6 if k == "x" else 1e-7), f"early rays: {k} rel err {v:.3e}"

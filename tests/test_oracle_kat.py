@@ -413,3 +413,44 @@ def test_superposition_mirror_camera():
     assert np.abs(X[3][ok] - X[3][:, ::-1][ok]).max() < 1e-6
     g = res.g.reshape(n, n)
     assert np.abs(g - g[:, ::-1]).max() < 1e-7
+
+
+# ------------------------------------------- Carter test (RT_TERM_CARTER_CAPTURE)
+def test_carter_verdict_matches_sampled_potential():
+    """The closed-form test (critical points of R by the cubic formula) against
+    R sampled on a fine grid outside r_+: no root <=> verdict 1, a root <=> 0."""
+    rng = np.random.default_rng(31)
+    decided = 0
+    for _ in range(3000):
+        a = float(rng.uniform(-0.998, 0.998))
+        xi = float(rng.uniform(-9, 9))
+        eta = float(rng.uniform(-1, 40))
+        v = O.oracle().rto_carter_verdict(1.0, a, xi, eta, 1e-6)
+        cap, margin = carter_captured(xi, eta, a)
+        if v == -1:
+            assert margin < 1e-3, (a, xi, eta, margin)
+            continue
+        decided += 1
+        assert (v == 1) == cap, (a, xi, eta, v, cap)
+    assert decided > 2900
+
+
+@pytest.mark.parametrize("a,incl,disk", [(0.998, 60.0, True), (0.5, 80.0, False), (0.0, 30.0, True)])
+def test_carter_capture_only_ends_captured_rays(a, incl, disk):
+    """With RT_TERM_CARTER_CAPTURE every ray the test ends is one that the full
+    integration captures; every other ray's result is unchanged."""
+    met = rtsim.kerr_metric(1.0, a)
+    tm = rtsim.default_termination(disk_enabled=disk)
+    tmc = rtsim.default_termination(disk_enabled=disk)
+    tmc.flags |= rtsim.RT_TERM_CARTER_CAPTURE
+    n = 24
+    cam = rtsim.default_camera(1000.0, math.radians(incl), 0.0, math.radians(1.6), n, n)
+    full = O.trace_camera(cam, met, rtsim.default_integrator(), tm)
+    fast = O.trace_camera(cam, met, rtsim.default_integrator(), tmc)
+    early = (fast.cls & rtsim.RT_FLAG_CARTER) != 0
+    assert early.sum() >= max(5, 0.5 * (full.klass == 1).sum()), (early.sum(), (full.klass == 1).sum())
+    assert np.all(full.klass[early] == 1) and np.all(fast.klass[early] == 1)
+    assert np.array_equal(fast.klass, full.klass)
+    same = ~early
+    assert np.array_equal(fast.steps[same], full.steps[same]) and np.array_equal(fast.x[:, same], full.x[:, same])
+    assert fast.steps[early].sum() < 0.2 * full.steps[early].sum()
