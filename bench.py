@@ -10,9 +10,11 @@ batch (rt_trace_batch, one persistent kernel, replayed from a hipGraph);
 --workload uneven512 times the 512-request batched workload instead.
 
 Prints one JSON line. value = rays/s over the timed steps. roofline: the
-tracing kernel's algorithmic FP64 FLOP per launch (right-hand-side
-evaluations x FLOP_RHS + attempted steps x FLOP_ATTEMPT, op-counted on the CPU
-oracle) over its HIP-event duration, against the MI355X FP64 vector peak.
+algorithmic FP64 FLOP per launch (right-hand-side evaluations x FLOP_RHS +
+attempted steps x FLOP_ATTEMPT, op-counted on the CPU oracle) over the
+launch's HIP-event duration on its stream, against the MI355X FP64 vector
+peak. A batch is one launch: one replay of its captured graph, whose per-frame
+kernels run as concurrent branches.
 cpu_baseline: the CPU oracle (a clean-room C++ port, FP64) on every k-th row
 of the same frames, all host threads.
 """
@@ -50,7 +52,7 @@ PEAK_FP64_TFLOPS = 78.6
 # device-to-device copy of known size (x2.000 and x1.000), over the rays of
 # the dispatch (4 x 2048^2 Kerr frames). A camera ray reads nothing and
 # writes 116 B of results.
-HBM_BYTES_PER_RAY = {"batch": 116.55, "plain": 119.86}
+HBM_BYTES_PER_RAY = {"k_trace_camera_kerr": 119.86, "k_trace_tiles": 116.55, "k_trace_queue": 116.55}
 METRIC = "rays/s Kerr 2048\u00b2 mixed vs FP64 at \u22641e-9 rel err; p50/p99 latency @QPS"
 
 
@@ -255,7 +257,7 @@ def main() -> int:
         # autotune the work-queue launch shape on every 16th request
         best = ctx.autotune([rtsim.make_request(*r) for r in reqs[::16]])
         tuned = {"block": best.block, "waves": best.waves, "refill_min": best.refill_min,
-                 "lane_refill": best.lane_refill}
+                 "queue_kind": best.queue_kind}
         print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
     print(f"bench: {len(reqs)} frames of {args.side}^2, precision {args.precision}, "
           f"warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
@@ -296,6 +298,12 @@ def main() -> int:
         print(f"bench: step {s + 1} done", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     value = rays / wall
+    # the tracing kernel of the timed launches: per-frame kernels for the plain
+    # path and for batches of queue_kind 2 (one per request, concurrent
+    # streams), else the persistent queue kernel of the batch
+    _, qs = ctx.launch_shapes()
+    kernel = ("k_trace_camera_kerr" if args.path == "plain" or qs.queue_kind == 2
+              else ("k_trace_queue" if qs.queue_kind == 1 else "k_trace_tiles"))
     avg_kernel_s = kernel_ms / launches / 1e3
     achieved = flops / launches / avg_kernel_s / 1e12
     out = {
@@ -317,10 +325,10 @@ def main() -> int:
                    "mean_accepted_steps": acc / rays, "mean_rejected_steps": rej / rays,
                    "rhs_evals_per_ray": rhs / rays, "path": args.path, "kernel_ms_per_launch": kernel_ms / launches,
                    "launches_per_step": launches // args.steps, "block": shape[0], "grid": shape[1],
-                   "autotuned": tuned},
+                   "autotuned": tuned, "kernel": kernel},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": HBM_BYTES_PER_RAY[args.path] * rays / launches},
+                     "traffic": HBM_BYTES_PER_RAY[kernel] * rays / launches},
     }
     ctx.close()
     if not args.no_serve:

@@ -248,8 +248,11 @@ typedef struct rt_launch_shape {
     uint32_t block;       /* threads per workgroup: 64, 128, 192 or 256          */
     uint32_t waves;       /* occupancy the kernel is built for: 1 or 2 waves/SIMD */
     uint32_t refill_min;  /* lane-refill queue: idle lanes that trigger a refill (1..64) */
-    uint32_t lane_refill; /* work queue: 0 = a wave takes whole 8x8 tiles,
-                             1 = idle lanes are refilled ray by ray           */
+    uint32_t queue_kind;  /* batches: 0 = one persistent kernel, a wave takes
+                             whole 8x8 tiles from a shared queue;
+                             1 = one persistent kernel, idle lanes refilled ray
+                             by ray; 2 = one per-frame kernel per request, the
+                             requests spread over concurrent streams          */
 } rt_launch_shape;
 
 /* Shapes used by the single-Kerr per-frame kernels (plain) and work-queue

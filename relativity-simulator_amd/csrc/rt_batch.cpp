@@ -160,12 +160,37 @@ rt_status upload_batch(const BatchPlan& plan, const Placed& pl, hipStream_t s) {
     return RT_OK;
 }
 
+// Streams a batch may fork onto (queue_kind 2): side streams and events.
+struct Branches {
+    const hipStream_t* side = nullptr;
+    hipEvent_t fork = nullptr;
+    const hipEvent_t* join = nullptr;
+    int n = 0;
+};
+
 // The launch sequence of a batch (captured into a graph by rt_trace_batch).
+// queue_kind 0 and 1: one persistent kernel over the shared queue.
+// queue_kind 2: one per-frame kernel (kPlainShape) per request, spread over s
+// and the side streams; captured, the streams become parallel graph branches,
+// so one request's slow rays overlap the other requests.
 hipError_t launch_batch(const BatchPlan& plan, const Placed& pl, const LaunchShape& shape, uint32_t grid,
-                        hipStream_t s) {
+                        hipStream_t s, const Branches& br) {
     hipError_t e = launch_zero_u64(pl.counters, static_cast<uint32_t>(pl.n_zero), s);
     if (e != hipSuccess) return e;
-    if (!plan.q_params.empty()) {
+    if (!plan.q_params.empty() && shape.queue_kind == 2 && br.n > 0) {
+        e = hipEventRecord(br.fork, s);
+        for (int i = 0; i < br.n && e == hipSuccess; ++i) e = hipStreamWaitEvent(br.side[i], br.fork, 0);
+        for (size_t j = 0; j < plan.q_params.size() && e == hipSuccess; ++j) {
+            const size_t b = j % static_cast<size_t>(br.n + 1);
+            e = launch_trace_camera(plan.q_params[j], pl.outs[plan.q_req[j]], kPlainShape,
+                                    b == 0 ? s : br.side[b - 1]);
+        }
+        for (int i = 0; i < br.n && e == hipSuccess; ++i) {
+            e = hipEventRecord(br.join[i], br.side[i]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, br.join[i], 0);
+        }
+        if (e != hipSuccess) return e;
+    } else if (!plan.q_params.empty()) {
         e = launch_trace_queue(pl.Q, shape, plan.mixed, plan.carter, grid, s);
         if (e != hipSuccess) return e;
     }
@@ -185,6 +210,11 @@ std::vector<unsigned char> graph_key(const BatchPlan& plan, const Placed& pl, co
     };
     put(&pl.Q, sizeof(pl.Q));
     put(&shape, sizeof(shape));
+    if (shape.queue_kind == 2)  // per-request kernels take their parameters by value
+        for (size_t j = 0; j < plan.q_params.size(); ++j) {
+            put(&plan.q_params[j], sizeof(TraceParams));
+            put(&pl.outs[plan.q_req[j]], sizeof(OutPtrs));
+        }
     const bool variant[2] = {plan.mixed, plan.carter};
     put(variant, sizeof(variant));
     for (size_t j = 0; j < plan.m_params.size(); ++j) {
@@ -235,7 +265,7 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
     const auto t0 = std::chrono::steady_clock::now();
     RT_HIP(hipSetDevice(ctx->device));
     BatchPlan plan;
-    rt_status st = plan_batch(reqs, n_req, plan, ctx->qshape.lane_refill == 0);
+    rt_status st = plan_batch(reqs, n_req, plan, ctx->qshape.queue_kind != 1);
     if (st != RT_OK) return st;
     st = ctx->arena.ensure(std::max<size_t>(plan.result_bytes, 256));
     if (st != RT_OK) return st;
@@ -253,7 +283,12 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
         uint32_t grid = 0;
         RT_HIP(queue_grid(ctx->qshape, plan.mixed, plan.carter, &grid));
         RT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        const hipError_t le = launch_batch(plan, pl, ctx->qshape, grid, ctx->stream);
+        Branches br;
+        br.side = ctx->side;
+        br.fork = ctx->fork;
+        br.join = ctx->join;
+        br.n = rt_context::kSide;
+        const hipError_t le = launch_batch(plan, pl, ctx->qshape, grid, ctx->stream, br);
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
         if (le != hipSuccess) {
@@ -374,7 +409,7 @@ struct rt_server {
     PoolAlloc alloc;
     std::vector<hipStream_t> streams;
     std::vector<bool> stream_busy;
-    LaunchShape qshape = kQueueShape;
+    LaunchShape qshape = kServerShape;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
@@ -444,7 +479,7 @@ void rt_server::launch_locked(std::unique_lock<std::mutex>& lk) {
     lk.unlock();
     // plan + place + upload + launch outside the lock
     BatchPlan plan;
-    rt_status st = plan_batch(batch.data(), batch.size(), plan, qshape.lane_refill == 0);
+    rt_status st = plan_batch(batch.data(), batch.size(), plan, qshape.queue_kind != 1);
     hipError_t he = hipSuccess;
     if (st == RT_OK) {
         char* base = static_cast<char*>(pool) + b->off;
@@ -452,7 +487,7 @@ void rt_server::launch_locked(std::unique_lock<std::mutex>& lk) {
         b->n_rays = plan.n_rays;
         hipStream_t s = streams[sidx];
         st = upload_batch(plan, b->pl, s);
-        if (st == RT_OK) he = launch_batch(plan, b->pl, qshape, 0, s);
+        if (st == RT_OK) he = launch_batch(plan, b->pl, qshape, 0, s, Branches{});
         if (st == RT_OK && he == hipSuccess) he = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
         if (st == RT_OK && he == hipSuccess) he = hipEventRecord(b->done, s);
     }
@@ -549,24 +584,24 @@ extern "C" {
 
 rt_status rt_get_launch_shape(const rt_context* ctx, rt_launch_shape* plain, rt_launch_shape* queue) {
     if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
-    if (plain) *plain = rt_launch_shape{ctx->shape.block, ctx->shape.waves, ctx->shape.refill_min, ctx->shape.lane_refill};
+    if (plain) *plain = rt_launch_shape{ctx->shape.block, ctx->shape.waves, ctx->shape.refill_min, ctx->shape.queue_kind};
     if (queue) *queue = rt_launch_shape{ctx->qshape.block, ctx->qshape.waves, ctx->qshape.refill_min,
-                                        ctx->qshape.lane_refill};
+                                        ctx->qshape.queue_kind};
     return RT_OK;
 }
 
 static bool shape_valid(const rt_launch_shape* s) {
     return s->block >= 64 && s->block <= 256 && (s->block % 64) == 0 && (s->waves == 1 || s->waves == 2) &&
-           s->refill_min >= 1 && s->refill_min <= 64 && s->lane_refill <= 1;
+           s->refill_min >= 1 && s->refill_min <= 64 && s->queue_kind <= 2;
 }
 
 rt_status rt_set_launch_shape(rt_context* ctx, const rt_launch_shape* plain, const rt_launch_shape* queue) {
     if (!ctx) return fail(RT_ERR_INVALID_ARG, "null context");
     if ((plain && !shape_valid(plain)) || (queue && !shape_valid(queue)))
-        return fail(RT_ERR_INVALID_ARG, "launch shape: block in {64,128,192,256}, waves in {1,2}, refill_min 1..64, lane_refill in {0,1}");
+        return fail(RT_ERR_INVALID_ARG, "launch shape: block in {64,128,192,256}, waves in {1,2}, refill_min 1..64, queue_kind in {0,1,2}");
     std::lock_guard<std::mutex> lk(ctx->mu);
-    if (plain) ctx->shape = LaunchShape{plain->block, plain->waves, plain->refill_min, plain->lane_refill};
-    if (queue) ctx->qshape = LaunchShape{queue->block, queue->waves, queue->refill_min, queue->lane_refill};
+    if (plain) ctx->shape = LaunchShape{plain->block, plain->waves, plain->refill_min, plain->queue_kind};
+    if (queue) ctx->qshape = LaunchShape{queue->block, queue->waves, queue->refill_min, queue->queue_kind};
     return RT_OK;
 }
 
@@ -582,7 +617,8 @@ rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample
         if (sample[i].metric.n_sources >= 2) return fail(RT_ERR_INVALID_ARG, "autotune takes single-Kerr requests");
     std::lock_guard<std::mutex> lk(ctx->mu);
     // candidates: tile queue at each block size and occupancy, lane-refill
-    // queue at each block size with three refill thresholds
+    // queue at each block size with three refill thresholds, per-request
+    // kernels on concurrent streams
     LaunchShape keep = ctx->qshape, win = ctx->qshape;
     std::vector<LaunchShape> cands;
     for (uint32_t b : {64u, 128u, 256u})
@@ -590,6 +626,7 @@ rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample
             cands.push_back(LaunchShape{b, w, 16, 0});
             for (uint32_t r : {8u, 16u, 32u}) cands.push_back(LaunchShape{b, w, r, 1});
         }
+    cands.push_back(LaunchShape{kPlainShape.block, kPlainShape.waves, 16, 2});
     double best_ms = 1e300;
     for (const LaunchShape& c : cands) {
         ctx->qshape = c;
@@ -606,7 +643,7 @@ rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample
     }
     ctx->qshape = win;
     ctx->tuned = true;
-    if (best) *best = rt_launch_shape{win.block, win.waves, win.refill_min, win.lane_refill};
+    if (best) *best = rt_launch_shape{win.block, win.waves, win.refill_min, win.queue_kind};
     return RT_OK;
 }
 

@@ -616,6 +616,11 @@ rt_status rt_context_create(int device, rt_context** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    for (int i = 0; i < rt_context::kSide && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
     if (e != hipSuccess) {
         rt_context_destroy(ctx);
         return hip_fail(e, "context set-up");
@@ -634,6 +639,11 @@ rt_status rt_context_destroy(rt_context* ctx) {
     ctx->pix.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (int i = 0; i < rt_context::kSide; ++i) {
+        if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
+        if (ctx->join[i]) (void)hipEventDestroy(ctx->join[i]);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RT_OK;

@@ -70,7 +70,13 @@ struct rt_context {
     size_t n_last = 0;
     rt_stats stats{};
     rt::LaunchShape shape;        // plain kernels
-    rt::LaunchShape qshape;       // work-queue kernel
+    rt::LaunchShape qshape;       // batches (work-queue kernels, or per-request kernels)
+    // Side streams and fork/join events: a batch with queue_kind 2 spreads its
+    // per-request kernels over the context stream and these (graph branches).
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kSide] = {nullptr, nullptr, nullptr};
     bool tuned = false;
     std::mutex mu;
 };

@@ -16,16 +16,23 @@ struct LaunchShape {
     uint32_t block = 64;
     uint32_t waves = 2;
     uint32_t refill_min = 16;
-    uint32_t lane_refill = 0;  // work queue: 0 whole 8x8 tiles per wave, 1 per-lane refill
+    uint32_t queue_kind = 0;   // batches: 0 tile queue, 1 lane-refill queue, 2 per-request kernels
 };
 
-// Defaults, the fastest shapes measured on MI355X: one-wave workgroups (the
-// hardware refills a wave slot as soon as its 8x8 tile is done) for the
-// single-Kerr kernels (4 x 2048^2 Kerr frames); for the superposition kernel
-// (2..16 sources, 512^2) two-wave occupancy with a refill once the whole wave
-// is idle.
+// Defaults, the fastest shapes measured on MI355X:
+//  - per-frame kernels: one-wave workgroups (the hardware refills a wave slot
+//    as soon as its 8x8 tile is done);
+//  - batches of a few large frames (4 x 2048^2 Kerr): one per-frame kernel per
+//    request on concurrent streams (queue_kind 2), 58 ms against 69.5 ms for
+//    the shared tile queue;
+//  - the serving loop's batches of many uneven requests: the shared tile queue
+//    at two waves per SIMD (512 uneven requests: 656 ms against 734 ms for
+//    per-request kernels);
+//  - the superposition kernel (2..16 sources, 512^2): two waves per SIMD, a
+//    refill once the whole wave is idle.
 constexpr LaunchShape kPlainShape{64, 1, 16, 0};
-constexpr LaunchShape kQueueShape{64, 1, 16, 0};
+constexpr LaunchShape kQueueShape{64, 1, 16, 2};
+constexpr LaunchShape kServerShape{64, 2, 16, 0};
 constexpr LaunchShape kMultiShape{64, 2, 64, 0};
 
 // Camera frame (single Kerr): one lane per pixel, lanes of a wave on an 8x8 tile.
@@ -38,7 +45,7 @@ uint32_t multi_ray_lanes(uint32_t n_src);
 hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
                               const LaunchShape& L, hipStream_t stream);
 // Persistent work-queue kernel over many single-Kerr requests (tiles or rays
-// per L.lane_refill; Q.req_offset counts tiles or rays accordingly).
+// per L.queue_kind 0 or 1; Q.req_offset counts tiles or rays accordingly).
 // grid 0: computed from the occupancy (not allowed inside a stream capture).
 hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixed, bool carter, uint32_t grid,
                               hipStream_t stream);

@@ -1298,7 +1298,7 @@ hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const Mult
 
 template <bool Mixed, bool Carter>
 static const void* queue_fn(const LaunchShape& L) {
-    if (L.lane_refill)
+    if (L.queue_kind == 1)
         return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_queue<Mixed, 2, Carter>)
                               : reinterpret_cast<const void*>(&k_trace_queue<Mixed, 1, Carter>);
     return (L.waves >= 2) ? reinterpret_cast<const void*>(&k_trace_tiles<Mixed, 2, Carter>)
@@ -1307,7 +1307,7 @@ static const void* queue_fn(const LaunchShape& L) {
 
 template <bool Mixed, bool Carter>
 static hipError_t launch_queue_t(const QueueDev& Q, const LaunchShape& L, uint32_t grid, hipStream_t stream) {
-    if (L.lane_refill) {
+    if (L.queue_kind == 1) {
         if (L.waves >= 2)
             hipLaunchKernelGGL((k_trace_queue<Mixed, 2, Carter>), dim3(grid), dim3(L.block), 0, stream, Q);
         else

@@ -113,7 +113,7 @@ class rt_request(C.Structure):
 
 
 class rt_launch_shape(C.Structure):
-    _fields_ = [("block", C.c_uint32), ("waves", C.c_uint32), ("refill_min", C.c_uint32), ("lane_refill", C.c_uint32)]
+    _fields_ = [("block", C.c_uint32), ("waves", C.c_uint32), ("refill_min", C.c_uint32), ("queue_kind", C.c_uint32)]
 
 
 class rt_server_config(C.Structure):

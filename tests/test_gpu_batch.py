@@ -40,10 +40,10 @@ def same(a, b, label):
     assert np.array_equal(a.steps, b.steps) and np.array_equal(a.rejected, b.rejected), f"{label}: step counts"
 
 
-@pytest.mark.parametrize("lane_refill", [0, 1], ids=["tile_queue", "lane_refill_queue"])
-def test_batch_matches_single_and_graph_replays(ctx, lane_refill):
+@pytest.mark.parametrize("kind", [0, 1, 2], ids=["tile_queue", "lane_refill_queue", "per_request_streams"])
+def test_batch_matches_single_and_graph_replays(ctx, kind):
     plain, queue = ctx.launch_shapes()
-    ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(128, 2, 16, lane_refill))
+    ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(128, 2, 16, kind))
     try:
         check_batch_replays(ctx)
     finally:
@@ -68,9 +68,9 @@ def check_batch_replays(ctx):
 def test_autotune_then_batch(ctx):
     reqs = uneven_requests()[:3]
     best = ctx.autotune(reqs)
-    print(f"autotuned block={best.block} waves={best.waves} refill={best.refill_min} lane_refill={best.lane_refill}",
+    print(f"autotuned block={best.block} waves={best.waves} refill={best.refill_min} queue_kind={best.queue_kind}",
           flush=True)
-    assert best.block in (64, 128, 256) and best.waves in (1, 2) and best.lane_refill in (0, 1)
+    assert best.block in (64, 128, 256) and best.waves in (1, 2) and best.queue_kind in (0, 1, 2)
     eager = [ctx.trace_camera(r.camera, r.metric, r.integrator, r.termination) for r in reqs]
     for o, e in zip(ctx.trace_batch(reqs), eager):
         same(o, e, "after autotune")
@@ -81,9 +81,9 @@ def test_launch_shape_roundtrip(ctx):
     s = rtsim.rt_launch_shape(128, 2, 8, 1)
     ctx.set_launch_shapes(queue=s)
     _, q2 = ctx.launch_shapes()
-    assert (q2.block, q2.waves, q2.refill_min, q2.lane_refill) == (128, 2, 8, 1)
+    assert (q2.block, q2.waves, q2.refill_min, q2.queue_kind) == (128, 2, 8, 1)
     with pytest.raises(rtsim.RtError):
         ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(100, 3, 0, 0))
     with pytest.raises(rtsim.RtError):
-        ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(64, 1, 16, 2))
+        ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(64, 1, 16, 3))
     ctx.set_launch_shapes(plain=plain, queue=queue)
