@@ -20,6 +20,10 @@
  *   rt_trace_batch       many frames in one dispatch: the tile/band loop of
  *                        TileRenderer::render (src/distributed/tile_renderer.cpp:133-182)
  *                        and VulkanComputeExecutor banding (vulkan_compute_executor.hpp:1394-1465)
+ *   rt_decode_tile_task  deserialize_render_tile_task + validate_render_tile_task
+ *                        (src/distributed/task_payloads.cpp:70-129)
+ *   rt_trace_tile        SoftwareComputeEngine::dispatch_tile
+ *                        (include/relativistic/render/software_compute_engine.hpp:905-1022)
  *   rt_server_*          the windowed worker of GeodesicComputePipeline
  *                        (include/relativistic/render/geodesic_compute_pipeline.hpp:529-728):
  *                        requests, cancellation, completion polling
@@ -242,6 +246,78 @@ typedef struct rt_pixel {
  * VulkanComputeExecutor::dispatch_and_readback (render/vulkan_compute_executor.hpp:1467-1600),
  * which fills a std::vector<GpuPixelOutput>. */
 rt_status rt_fetch_pixels(rt_context* ctx, rt_pixel* out);
+
+/* ------------------------------------------------------ 32x32 tile tasks
+ * The cluster's tile protocol (include/relativistic/distributed/task_payloads.hpp):
+ * a frame is cut into 32x32 tiles numbered row-major, the last column and row
+ * of tiles ragged; a worker receives a RenderTileTaskPayload and returns the
+ * tile's GpuPixelOutput records, tile-local row-major. */
+#define RT_TILE_EDGE 32u               /* kTileEdgePixels                      */
+#define RT_TILE_MAX_SCREEN_WIDTH 7680u /* kMaximumScreenWidth                  */
+#define RT_TILE_MAX_SCREEN_HEIGHT 4320u/* kMaximumScreenHeight                 */
+/* Wire size of one task (kRenderTileTaskPayloadBytes): the 16-byte
+ * RenderTileTaskHeader, the f64 dispatch timestamp, then the 11056-byte
+ * GpuCameraPushConstants (render/gpu_types.hpp:966-1109), little-endian. */
+#define RT_TILE_TASK_BYTES 11080u
+
+/* A rectangle of a frame (TileGeometry, task_payloads.hpp:42-51). */
+typedef struct rt_tile {
+    uint32_t origin_x, origin_y, width, height;
+} rt_tile;
+
+/* Geometry of tile tile_index of a screen_width x screen_height frame.
+ * Replaces resolve_tile_geometry (src/distributed/task_payloads.cpp:29-44):
+ * RT_ERR_INVALID_ARG where it returns nullopt (zero or oversized screen,
+ * index past the last tile). */
+rt_status rt_tile_geometry(uint32_t tile_index, uint32_t screen_width, uint32_t screen_height, rt_tile* out);
+
+/* A decoded tile task: the header fields, the resolved tile, and the push
+ * constants mapped onto this library's request (single Kerr; see
+ * rt_decode_tile_task). */
+typedef struct rt_tile_task {
+    uint32_t tile_index;          /* RenderTileTaskHeader fields                 */
+    uint16_t lod_reduced_steps;
+    uint8_t flags;                /* kTileTaskFlag* bits, passed through         */
+    uint8_t compression_type;     /* result TileCompressionMode, passed through  */
+    double dispatch_timestamp;
+    rt_tile tile;                 /* resolve_tile_geometry of the task           */
+    uint32_t metric_type;         /* 1 Schwarzschild, 2 or 3 Kerr                */
+    uint32_t body_count;          /* GpuBodyData records sent beside the task    */
+    rt_camera camera;             /* whole frame; the tile is the traced window  */
+    rt_metric metric;
+    rt_integrator integrator;
+    rt_termination termination;
+} rt_tile_task;
+
+/* Decode and validate n_bytes of wire data into *out. Replaces
+ * deserialize_render_tile_task + validate_render_tile_task
+ * (src/distributed/task_payloads.cpp:70-129): RT_ERR_INVALID_ARG exactly where
+ * they return false (short buffer, zero or oversized screen, zero step
+ * budget, non-finite observer / tetrad / field of view / mass / spin /
+ * escape radius / timestamp, unknown compression mode, header that does not
+ * match the tile's geometry), and also for a metric_type outside the Kerr
+ * family (1, 2, 3), which this tracer does not integrate. The push constants
+ * map as dispatch_fp64_scalar reads them (render/software_compute_engine.hpp:3794-3813):
+ * observer_position and the spatial parts of tetrad_e1/e2/e3 -> camera;
+ * metric_mass and metric_spin (0 for metric_type 1, clamped to +-0.999 M)
+ * -> metric; escape_radius -> termination.escape_radius;
+ * max_integration_steps -> integrator.max_steps (other integrator fields:
+ * rt_default_integrator); disk on when primary_disk.enabled > 0.5, inner
+ * radius ISCO x max(inner_radius_scale, 1), outer radius
+ * max(outer_radius_mass_units x M, 1.05 x inner). */
+rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* out);
+
+/* Trace the rectangle *region of the camera's frame (rows and columns of the
+ * full width x height frame; the camera's row window is ignored) and keep
+ * the results device-resident, tile-local row-major, for rt_fetch_results /
+ * rt_fetch_pixels; *n_rays receives width * height. Every pixel's result is
+ * the one rt_trace_camera gives for it in the whole frame. Replaces
+ * SoftwareComputeEngine::dispatch_tile (render/software_compute_engine.hpp:905-1022)
+ * as called by TileRenderer::render (src/distributed/tile_renderer.cpp:133-182);
+ * any rectangle inside the frame is accepted, 32x32 tiles being one case. */
+rt_status rt_trace_tile(rt_context* ctx, const rt_camera* cam, const rt_metric* metric,
+                        const rt_integrator* integ, const rt_termination* term, const rt_tile* region,
+                        size_t* n_rays);
 
 /* ------------------------------------------------------- launch shapes */
 typedef struct rt_launch_shape {

@@ -14,6 +14,7 @@
 #include "relativistic/optics/disk_thermal_profile.hpp"
 #include "relativistic/render/accretion_disk/accretion_disk_model.hpp"
 #include "relativistic/render/software_compute_engine.hpp"
+#include "relativistic/distributed/task_payloads.hpp"
 
 #include <span>
 #include <vector>
@@ -242,6 +243,93 @@ void ref_ms_initial_momentum(const rt_metric* mt, const double x[3], const doubl
     p[0] = q[0];
     p[1] = q[1];
     p[2] = q[2];
+}
+
+// ---- 32x32 tile tasks (src/distributed/task_payloads.cpp, linked from
+// the reference's own source by oracle/Makefile)
+
+// resolve_tile_geometry (task_payloads.cpp:29-44): 1 and out[4] = origin_x,
+// origin_y, width, height, or 0 where it returns nullopt.
+int ref_tile_geometry(uint32_t index, uint32_t w, uint32_t h, uint32_t out[4]) {
+    const auto g = Distributed::resolve_tile_geometry(index, w, h);
+    if (!g) return 0;
+    out[0] = g->origin_x;
+    out[1] = g->origin_y;
+    out[2] = g->width;
+    out[3] = g->height;
+    return 1;
+}
+
+// Sizes and the byte offsets (in the serialized task) of the fields a tile
+// worker reads: [0] task bytes, [1] header bytes, [2] push-constant offset,
+// [3] push-constant size, then observer_position, tetrad_e1, tetrad_e2,
+// tetrad_e3, field_of_view_rad, metric_mass, metric_spin, escape_radius,
+// screen_width, screen_height, metric_type, max_integration_steps,
+// body_count, primary_disk.enabled, .inner_radius_scale,
+// .outer_radius_mass_units (offsets from the start of the push constants).
+int ref_tile_layout(uint64_t* out) {
+    using Render::GpuCameraPushConstants;
+    using Render::GpuDiskProfile;
+    const uint64_t disk = offsetof(GpuCameraPushConstants, primary_disk);
+    const uint64_t v[] = {Distributed::kRenderTileTaskPayloadBytes, sizeof(Distributed::RenderTileTaskHeader),
+                          Distributed::kRenderTileTaskPrefixBytes, sizeof(GpuCameraPushConstants),
+                          offsetof(GpuCameraPushConstants, observer_position), offsetof(GpuCameraPushConstants, tetrad_e1),
+                          offsetof(GpuCameraPushConstants, tetrad_e2), offsetof(GpuCameraPushConstants, tetrad_e3),
+                          offsetof(GpuCameraPushConstants, field_of_view_rad), offsetof(GpuCameraPushConstants, metric_mass),
+                          offsetof(GpuCameraPushConstants, metric_spin), offsetof(GpuCameraPushConstants, escape_radius),
+                          offsetof(GpuCameraPushConstants, screen_width), offsetof(GpuCameraPushConstants, screen_height),
+                          offsetof(GpuCameraPushConstants, metric_type),
+                          offsetof(GpuCameraPushConstants, max_integration_steps),
+                          offsetof(GpuCameraPushConstants, body_count), disk + offsetof(GpuDiskProfile, enabled),
+                          disk + offsetof(GpuDiskProfile, inner_radius_scale),
+                          disk + offsetof(GpuDiskProfile, outer_radius_mass_units)};
+    const int n = static_cast<int>(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n; ++i) out[i] = v[i];
+    return n;
+}
+
+// A task with default push constants except the given fields, header from
+// make_render_tile_task_header, written by serialize_render_tile_task
+// (task_payloads.cpp:46-112). d: observer[4], e1[4], e2[4], e3[4], fov,
+// mass, spin, escape radius, timestamp; u: tile index, width, height,
+// lod_reduced_steps, flags, compression mode, max steps, metric type, body
+// count; f: disk enabled, inner radius scale, outer radius (M). Returns the
+// bytes written (0 where the reference refuses the task).
+size_t ref_make_tile_task(const double* d, const uint32_t* u, const float* f, uint8_t* out, size_t cap) {
+    Distributed::RenderTileTaskPayload task{};
+    auto& c = task.camera_constants;
+    for (int i = 0; i < 4; ++i) {
+        c.observer_position[i] = d[i];
+        c.tetrad_e1[i] = d[4 + i];
+        c.tetrad_e2[i] = d[8 + i];
+        c.tetrad_e3[i] = d[12 + i];
+    }
+    c.field_of_view_rad = d[16];
+    c.metric_mass = d[17];
+    c.metric_spin = d[18];
+    c.escape_radius = d[19];
+    task.dispatch_timestamp = d[20];
+    c.screen_width = u[1];
+    c.screen_height = u[2];
+    c.max_integration_steps = u[6];
+    c.metric_type = u[7];
+    c.body_count = u[8];
+    c.primary_disk.enabled = f[0];
+    c.primary_disk.inner_radius_scale = f[1];
+    c.primary_disk.outer_radius_mass_units = f[2];
+    const auto h = Distributed::make_render_tile_task_header(u[0], u[1], u[2], static_cast<uint16_t>(u[3]),
+                                                             static_cast<uint8_t>(u[4]),
+                                                             static_cast<Distributed::TileCompressionMode>(u[5]));
+    if (!h) return 0;
+    task.header = *h;
+    return Distributed::serialize_render_tile_task(task, std::span<uint8_t>(out, cap));
+}
+
+// deserialize_render_tile_task (task_payloads.cpp:114-129): 1 when it
+// accepts the bytes, 0 when it refuses them.
+int ref_accepts_tile_task(const uint8_t* bytes, size_t n) {
+    Distributed::RenderTileTaskPayload task{};
+    return Distributed::deserialize_render_tile_task(std::span<const uint8_t>(bytes, n), task) ? 1 : 0;
 }
 
 }  // extern "C"

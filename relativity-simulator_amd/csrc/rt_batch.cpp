@@ -73,7 +73,7 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool til
             fill_camera(&r.camera, P, nullptr);
             plan.q_req.push_back(static_cast<uint32_t>(i));
             plan.q_params.push_back(P);
-            const uint64_t tiles_y = (P.n_rays / P.width + 7) / 8;
+            const uint64_t tiles_y = (P.n_rays / P.cols + 7) / 8;
             const uint64_t n_tiles = static_cast<uint64_t>(P.tiles_x) * tiles_y;
             const uint64_t n_hot = static_cast<uint64_t>(P.hot_w) * P.hot_h;
             const uint64_t unit = tiles ? 1 : 64;

@@ -245,12 +245,12 @@ static void hot_tiles(TraceParams& P, double r_obs) {
     if (!(cf > 1e-6) || !(P.tan_half > 0.0)) return;  // hole behind the camera
     const double W = P.width, H = P.height;
     const double u = cr / (cf * P.tan_half), v = cu / (cf * P.tan_half);
-    const double px = (u / P.aspect + 1.0) * 0.5 * W - 0.5;
+    const double px = (u / P.aspect + 1.0) * 0.5 * W - 0.5 - static_cast<double>(P.col_first);
     const double py = (1.0 - v) * 0.5 * H - 0.5 - static_cast<double>(P.row_first);
     const double s = std::min(1.0, 10.0 * P.M / r_obs);
     const double alpha = std::asin(s);
     const double r_pix = (alpha >= 1.5) ? 1e9 : std::tan(alpha) / P.tan_half * 0.5 * H + 8.0;
-    const double tiles_y = static_cast<double>((P.n_rays / P.width + 7) / 8);
+    const double tiles_y = static_cast<double>((P.n_rays / P.cols + 7) / 8);
     const double x0 = std::max(0.0, std::floor((px - r_pix) / 8.0));
     const double x1 = std::min(static_cast<double>(P.tiles_x) - 1.0, std::floor((px + r_pix) / 8.0));
     const double y0 = std::max(0.0, std::floor((py - r_pix) / 8.0));
@@ -262,14 +262,23 @@ static void hot_tiles(TraceParams& P, double r_obs) {
     P.hot_h = static_cast<uint32_t>(y1 - y0 + 1.0);
 }
 
-void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T) {
-    const uint32_t first = std::min(c->row_offset, c->height);
-    const uint32_t last = c->row_count ? std::min(first + c->row_count, c->height) : c->height;
+void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T, const rt_tile* win) {
+    uint32_t first = std::min(c->row_offset, c->height);
+    uint32_t last = c->row_count ? std::min(first + c->row_count, c->height) : c->height;
+    uint32_t col0 = 0, cols = c->width;
+    if (win) {  // a rectangle of the frame (checked by valid_window)
+        first = win->origin_y;
+        last = win->origin_y + win->height;
+        col0 = win->origin_x;
+        cols = win->width;
+    }
     P.width = c->width;
     P.height = c->height;
     P.row_first = first;
-    P.n_rays = (last - first) * c->width;
-    P.tiles_x = (c->width + 7) / 8;
+    P.col_first = col0;
+    P.cols = cols;
+    P.n_rays = (last - first) * cols;
+    P.tiles_x = (cols + 7) / 8;
     P.mode = 0;
     P.aspect = static_cast<double>(c->width) / static_cast<double>(c->height);
     P.tan_half = std::tan(c->fov_y * 0.5);
@@ -467,9 +476,11 @@ namespace {
 
 // Trace one request (camera or explicit rays) on the context's stream into
 // the context arena; times the kernel with HIP events.
-// x_soa/p_soa non-null: explicit rays (n of them), else camera cam.
+// x_soa/p_soa non-null: explicit rays (n of them), else camera cam, over the
+// rectangle win of its frame when win is non-null.
 rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, const double* p_soa, size_t n_in,
-                    const rt_metric* metric, const rt_integrator* integ, const rt_termination* term) {
+                    const rt_metric* metric, const rt_integrator* integ, const rt_termination* term,
+                    const rt_tile* win = nullptr) {
     const auto t0 = std::chrono::steady_clock::now();
     RT_HIP(hipSetDevice(ctx->device));
     TraceParams P;
@@ -479,7 +490,7 @@ rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, 
     if (multi) fill_sources(metric, term, T);
     size_t n;
     if (cam) {
-        fill_camera(cam, P, multi ? &T : nullptr);
+        fill_camera(cam, P, multi ? &T : nullptr, win);
         n = P.n_rays;
     } else {
         P.mode = 1;
@@ -662,6 +673,22 @@ rt_status rt_trace_camera_resident(rt_context* ctx, const rt_camera* cam, const 
     if (!valid_request(metric, integ, term, why) || !valid_camera(cam, why)) return fail(RT_ERR_INVALID_ARG, why);
     std::lock_guard<std::mutex> lk(ctx->mu);
     rt_status st = trace_one(ctx, cam, nullptr, nullptr, 0, metric, integ, term);
+    if (st != RT_OK) return st;
+    if (n_rays) *n_rays = ctx->n_last;
+    return RT_OK;
+}
+
+rt_status rt_trace_tile(rt_context* ctx, const rt_camera* cam, const rt_metric* metric, const rt_integrator* integ,
+                        const rt_termination* term, const rt_tile* region, size_t* n_rays) {
+    if (!ctx || !region) return fail(RT_ERR_INVALID_ARG, "null context or region");
+    std::string why;
+    if (!valid_request(metric, integ, term, why) || !valid_camera(cam, why)) return fail(RT_ERR_INVALID_ARG, why);
+    if (region->width == 0 || region->height == 0 || region->origin_x >= cam->width ||
+        region->origin_y >= cam->height || region->width > cam->width - region->origin_x ||
+        region->height > cam->height - region->origin_y)
+        return fail(RT_ERR_INVALID_ARG, "region: empty, or not inside the camera's frame");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rt_status st = trace_one(ctx, cam, nullptr, nullptr, 0, metric, integ, term, region);
     if (st != RT_OK) return st;
     if (n_rays) *n_rays = ctx->n_last;
     return RT_OK;

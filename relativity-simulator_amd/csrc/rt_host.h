@@ -27,7 +27,7 @@ bool valid_camera(const rt_camera* c, std::string& why);
 void fill_params(const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm, TraceParams& P);
 // Camera constants; for a superposition also the source table and the
 // observer frame (T must be non-null then).
-void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T);
+void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T, const rt_tile* win = nullptr);
 // Source table of a superposition (sources only, no observer frame).
 void fill_sources(const rt_metric* mt, const rt_termination* tm, MultiDev& T);
 double kerr_isco_host(double m, double a_spin);

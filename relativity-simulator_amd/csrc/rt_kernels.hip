@@ -412,17 +412,17 @@ __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P,
     tile_xy(P, static_cast<uint32_t>(t >> 6), tx, ty);
     const uint32_t px = tx * 8 + (lane & 7);
     const uint32_t py = ty * 8 + (lane >> 3);
-    const uint32_t rows = P.n_rays / P.width;
+    const uint32_t rows = P.n_rays / P.cols;
     Tally tally;
-    if (px < P.width && py < rows) {
+    if (px < P.cols && py < rows) {
         double y0[8], xi;
-        camera_state_kerr(P, px, P.row_first + py, y0, &xi);
+        camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
         Lane L;
         lane_begin<Carter>(P, y0, xi, L);
         while (!lane_step<Mixed>(P, L)) {
         }
         lane_finish(L);
-        store_lane(O, static_cast<uint64_t>(py) * P.width + px, L, P.a, P.a2);
+        store_lane(O, static_cast<uint64_t>(py) * P.cols + px, L, P.a, P.a2);
         tally.add(L);
     }
     tally.flush(O.counters);
@@ -495,10 +495,10 @@ __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
                         tile_xy(P, static_cast<uint32_t>(ray >> 6) + (hot ? 0u : P.hot_w * P.hot_h), tx, ty);
                         const uint32_t px = tx * 8 + (l & 7);
                         const uint32_t py = ty * 8 + (l >> 3);
-                        ok = (px < P.width) && (py < P.n_rays / P.width);
+                        ok = (px < P.cols) && (py < P.n_rays / P.cols);
                         if (ok) {
-                            camera_state_kerr(P, px, P.row_first + py, y0, &xi);
-                            ray = static_cast<uint64_t>(py) * P.width + px;
+                            camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
+                            ray = static_cast<uint64_t>(py) * P.cols + px;
                         }
                     } else {
                         const OutPtrs& O = Q.outs[req];
@@ -569,15 +569,15 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
         const uint32_t px = tx * 8 + (lane & 7);
         const uint32_t py = ty * 8 + (lane >> 3);
         uint32_t c_acc = 0, c_rej = 0, c_rhs = 0, c_f32 = 0, c_proj = 0;
-        if (px < P.width && py < P.n_rays / P.width) {
+        if (px < P.cols && py < P.n_rays / P.cols) {
             double y0[8], xi = 0.0;
-            camera_state_kerr(P, px, P.row_first + py, y0, &xi);
+            camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
             Lane L;
             lane_begin<Carter>(P, y0, xi, L);
             while (!lane_step<Mixed>(P, L)) {
             }
             lane_finish(L);
-            store_lane(outs[req], static_cast<uint64_t>(py) * P.width + px, L, P.a, P.a2);
+            store_lane(outs[req], static_cast<uint64_t>(py) * P.cols + px, L, P.a, P.a2);
             c_acc = L.acc;
             c_rej = L.rej;
             c_rhs = L.rhs;
@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
     const uint32_t lane0 = lane_id & ~static_cast<uint32_t>(G - 1);
     const uint64_t below = (lane0 == 0) ? 0ull : (~0ull >> (64 - lane0));
     const GroupSrc gs{&sm.src[si < T.n ? si : 0], si < T.n};
-    const uint64_t total = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.width + 7) / 8) * 64
+    const uint64_t total = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.cols + 7) / 8) * 64
                                          : static_cast<uint64_t>(P.n_rays);
     Lane L;
     bool active = false, drained = false;
@@ -1165,10 +1165,10 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
                         tile_xy(P, static_cast<uint32_t>(g >> 6), tx, ty);
                         const uint32_t px = tx * 8 + (l & 7);
                         const uint32_t py = ty * 8 + (l >> 3);
-                        ok = (px < P.width) && (py < P.n_rays / P.width);
+                        ok = (px < P.cols) && (py < P.n_rays / P.cols);
                         if (ok) {
-                            camera_state_multi(P, T, px, P.row_first + py, y0);
-                            idx = static_cast<uint64_t>(py) * P.width + px;
+                            camera_state_multi(P, T, P.col_first + px, P.row_first + py, y0);
+                            idx = static_cast<uint64_t>(py) * P.cols + px;
                         }
                     } else {
                         idx = g;
@@ -1216,7 +1216,7 @@ static bool shape_ok(const LaunchShape& L) { return L.block != 0 && L.block <= 2
 
 template <bool Mixed, bool Carter>
 static hipError_t launch_camera_t(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
-    const uint64_t tiles_y = (P.n_rays / P.width + 7) / 8;
+    const uint64_t tiles_y = (P.n_rays / P.cols + 7) / 8;
     const uint64_t threads = static_cast<uint64_t>(P.tiles_x) * tiles_y * 64;
     const uint32_t grid = static_cast<uint32_t>((threads + L.block - 1) / L.block);
     if (L.waves >= 2)
@@ -1227,7 +1227,7 @@ static hipError_t launch_camera_t(const TraceParams& P, const OutPtrs& O, const 
 }
 
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
-    if (!shape_ok(L) || P.width == 0 || P.n_rays % P.width != 0) return hipErrorInvalidValue;
+    if (!shape_ok(L) || P.cols == 0 || P.n_rays % P.cols != 0) return hipErrorInvalidValue;
     if (P.carter)
         return P.mixed ? launch_camera_t<true, true>(P, O, L, stream) : launch_camera_t<false, true>(P, O, L, stream);
     return P.mixed ? launch_camera_t<true, false>(P, O, L, stream) : launch_camera_t<false, false>(P, O, L, stream);
@@ -1269,7 +1269,7 @@ static hipError_t launch_multi_g(const TraceParams& P, const OutPtrs& O, const M
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, static_cast<int>(L.block), 0);
     if (e != hipSuccess) return e;
     // no more lanes than G per ray (a small request leaves the rest of the device to others)
-    const uint64_t rays = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.width + 7) / 8) * 64
+    const uint64_t rays = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.cols + 7) / 8) * 64
                                         : static_cast<uint64_t>(P.n_rays);
     const uint64_t want = (rays * G + L.block - 1) / L.block;
     const uint32_t grid = static_cast<uint32_t>(
@@ -1285,7 +1285,7 @@ static hipError_t launch_multi_g(const TraceParams& P, const OutPtrs& O, const M
 hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
                               const LaunchShape& L, hipStream_t stream) {
     if (!shape_ok(L)) return hipErrorInvalidValue;
-    if (P.mode == 0 && (P.width == 0 || P.n_rays % P.width != 0)) return hipErrorInvalidValue;
+    if (P.mode == 0 && (P.cols == 0 || P.n_rays % P.cols != 0)) return hipErrorInvalidValue;
     if (P.n_src < 2 || P.n_src > static_cast<uint32_t>(kMaxSources)) return hipErrorInvalidValue;
     switch (multi_ray_lanes(P.n_src)) {
         case 2: return launch_multi_g<2>(P, O, T, next, L, stream);

@@ -27,8 +27,11 @@ struct TraceParams {
     uint32_t max_steps;
     uint32_t precision; // rt_precision
     // ---- camera (per-frame constants; identical for every pixel)
-    uint32_t width, height, row_first, n_rays;
-    uint32_t tiles_x;   // 8x8 pixel tiles per row of tiles
+    uint32_t width, height;    // frame size (the pixel -> screen mapping)
+    // Traced window: rows row_first.., columns col_first..col_first+cols-1;
+    // n_rays = rows * cols, results row-major over the window.
+    uint32_t row_first, col_first, cols, n_rays;
+    uint32_t tiles_x;   // 8x8 pixel tiles per row of tiles of the window
     uint32_t mode;      // 0 camera, 1 explicit rays
     // Tile order: the rectangle of tiles around the hole's image (the
     // captured and near-critical rays, which take up to ~100x the steps of

@@ -95,6 +95,25 @@ PIXEL_DTYPE = np.dtype([("r", np.float32), ("g", np.float32), ("b", np.float32),
                         ("iterations_used", np.uint32)])
 
 
+RT_TILE_EDGE = 32
+RT_TILE_MAX_SCREEN_WIDTH = 7680
+RT_TILE_MAX_SCREEN_HEIGHT = 4320
+RT_TILE_TASK_BYTES = 11080
+
+
+class rt_tile(C.Structure):
+    """A rectangle of a frame (the reference's TileGeometry)."""
+    _fields_ = [("origin_x", C.c_uint32), ("origin_y", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class rt_tile_task(C.Structure):
+    """A decoded RenderTileTaskPayload (rt_decode_tile_task)."""
+    _fields_ = [("tile_index", C.c_uint32), ("lod_reduced_steps", C.c_uint16), ("flags", C.c_uint8),
+                ("compression_type", C.c_uint8), ("dispatch_timestamp", C.c_double), ("tile", rt_tile),
+                ("metric_type", C.c_uint32), ("body_count", C.c_uint32), ("camera", rt_camera),
+                ("metric", rt_metric), ("integrator", rt_integrator), ("termination", rt_termination)]
+
+
 class rt_results(C.Structure):
     _fields_ = [("cls", C.POINTER(C.c_uint32)), ("x", C.POINTER(C.c_double)), ("p", C.POINTER(C.c_double)),
                 ("bl", C.POINTER(C.c_double)), ("lam", C.POINTER(C.c_double)), ("g", C.POINTER(C.c_double)),
@@ -238,7 +257,7 @@ EXPORTS = [
     "rt_default_integrator", "rt_default_termination", "rt_default_camera", "rt_kerr_metric",
     "rt_abi_version", "rt_version", "rt_last_error", "rt_context_create", "rt_context_destroy",
     "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
-    "rt_fetch_pixels",
+    "rt_fetch_pixels", "rt_tile_geometry", "rt_decode_tile_task", "rt_trace_tile",
     "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
     "rt_server_wait", "rt_server_take", "rt_server_cancel",
 ]
@@ -268,6 +287,10 @@ def load(path: Optional[str] = None) -> C.CDLL:
                                              P(rt_termination), P(C.c_size_t)]
     lib.rt_fetch_results.argtypes = [C.c_void_p, P(rt_results)]
     lib.rt_fetch_pixels.argtypes = [C.c_void_p, C.c_void_p]
+    lib.rt_tile_geometry.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, P(rt_tile)]
+    lib.rt_decode_tile_task.argtypes = [C.c_void_p, C.c_size_t, P(rt_tile_task)]
+    lib.rt_trace_tile.argtypes = [C.c_void_p, P(rt_camera), P(rt_metric), P(rt_integrator), P(rt_termination),
+                                  P(rt_tile), P(C.c_size_t)]
     lib.rt_trace_batch.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_results)]
     lib.rt_get_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
     lib.rt_set_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
@@ -292,6 +315,26 @@ def load(path: Optional[str] = None) -> C.CDLL:
 def _check(lib, status: int) -> None:
     if status != 0:
         raise RtError(status, lib.rt_last_error().decode())
+
+
+def tile_geometry(tile_index: int, screen_width: int, screen_height: int) -> rt_tile:
+    """Tile tile_index of a frame (resolve_tile_geometry); RtError where the
+    reference returns nullopt. Host-only: needs no device."""
+    lib = load()
+    t = rt_tile()
+    _check(lib, lib.rt_tile_geometry(tile_index, screen_width, screen_height, C.byref(t)))
+    return t
+
+
+def decode_tile_task(data: bytes) -> rt_tile_task:
+    """Decode and validate a RenderTileTaskPayload's wire bytes; RtError where
+    the reference's deserialize_render_tile_task returns false (and for a
+    metric outside the Kerr family). Host-only: needs no device."""
+    lib = load()
+    buf = C.create_string_buffer(bytes(data), len(data))
+    t = rt_tile_task()
+    _check(lib, lib.rt_decode_tile_task(buf, len(data), C.byref(t)))
+    return t
 
 
 class Context:
@@ -330,6 +373,15 @@ class Context:
         n = C.c_size_t()
         _check(self.lib, self.lib.rt_trace_camera_resident(self.h, C.byref(cam), C.byref(metric),
                                                            C.byref(integ), C.byref(term), C.byref(n)))
+        return n.value
+
+    def trace_tile(self, cam: rt_camera, metric: rt_metric, integ: rt_integrator, term: rt_termination,
+                   tile: rt_tile) -> int:
+        """Trace the rectangle tile of cam's frame; results stay resident
+        (tile-local row-major) for fetch / fetch_pixels. Returns the ray count."""
+        n = C.c_size_t()
+        _check(self.lib, self.lib.rt_trace_tile(self.h, C.byref(cam), C.byref(metric), C.byref(integ),
+                                                C.byref(term), C.byref(tile), C.byref(n)))
         return n.value
 
     def fetch(self, n: int) -> Results:

@@ -32,7 +32,7 @@ def test_every_declared_symbol_is_exported():
 
 
 STRUCTS = ["rt_source", "rt_metric", "rt_integrator", "rt_termination", "rt_camera", "rt_results", "rt_stats",
-           "rt_request", "rt_server_config", "rt_launch_shape", "rt_pixel"]
+           "rt_request", "rt_server_config", "rt_launch_shape", "rt_pixel", "rt_tile", "rt_tile_task"]
 
 
 def test_struct_layout_matches_ctypes():
@@ -41,7 +41,8 @@ def test_struct_layout_matches_ctypes():
         src += f'printf("{s} %zu\\n", sizeof({s}));\n'
     for s, f in [("rt_integrator", "max_steps"), ("rt_camera", "width"), ("rt_metric", "src"),
                  ("rt_request", "termination"), ("rt_stats", "block_size"), ("rt_results", "rejected"),
-                 ("rt_pixel", "redshift"), ("rt_pixel", "iterations_used")]:
+                 ("rt_pixel", "redshift"), ("rt_pixel", "iterations_used"), ("rt_tile_task", "dispatch_timestamp"),
+                 ("rt_tile_task", "camera"), ("rt_tile_task", "metric"), ("rt_tile_task", "termination")]:
         src += f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));\n'
     src += "return 0;}\n"
     with tempfile.TemporaryDirectory() as d:
@@ -64,6 +65,8 @@ def test_struct_layout_matches_ctypes():
     assert int(got["rt_pixel"]) == 32 == rtsim.PIXEL_DTYPE.itemsize
     assert int(got["rt_pixel.redshift"]) == 16 == rtsim.PIXEL_DTYPE.fields["redshift"][1]
     assert int(got["rt_pixel.iterations_used"]) == 28 == rtsim.PIXEL_DTYPE.fields["iterations_used"][1]
+    for f in ["dispatch_timestamp", "camera", "metric", "termination"]:
+        assert int(got[f"rt_tile_task.{f}"]) == getattr(rtsim.rt_tile_task, f).offset, f
 
 
 def test_defaults_match_python_mirror():

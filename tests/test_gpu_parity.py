@@ -54,6 +54,20 @@ def test_camera_vs_oracle(ctx, spin, incl):
     compare(gpu, ref, f"camera a={spin} i={incl}")
 
 
+def test_reference_cpu_configuration(ctx):
+    """The CPU-runnable configuration at its full size: M = 1, a = 0.9,
+    r_obs = 1000 M, inclination 80 deg, 256 x 256 pinhole camera with a 20 deg
+    field of view, rtol 1e-10, atol 1e-12, all 65 536 rays against the oracle."""
+    cam = rtsim.default_camera(1000.0, math.radians(80.0), 0.0, math.radians(20.0), 256, 256)
+    met = rtsim.kerr_metric(1.0, 0.9)
+    ig = rtsim.default_integrator(rtol=1e-10, atol=1e-12)
+    tm = rtsim.default_termination()
+    gpu = ctx.trace_camera(cam, met, ig, tm)
+    ref = O.trace_camera(cam, met, ig, tm)
+    assert {1, 2, 4} <= set(ref.klass.tolist())
+    compare(gpu, ref, "256^2 a=0.9 i=80")
+
+
 def test_camera_center_zoom(ctx):
     """Narrow field on the shadow edge: captured, disk and escaped rays mixed."""
     cam = rtsim.default_camera(1000.0, math.radians(60.0), 0.0, math.radians(1.6), 40, 40)

@@ -1,0 +1,140 @@
+"""The 32x32 tile task boundary on the host (no GPU): tile geometry and the
+decoding of RenderTileTaskPayload wire bytes, against the reference's own
+task_payloads.cpp — through the committed fixtures (tests/golden/tile_tasks.json,
+written by tests/golden/make_tile_tasks.py) and, where oracle/_ref is built,
+directly on mutated tasks."""
+import base64
+import ctypes as C
+import json
+import math
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import rtsim
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = json.load(open(os.path.join(HERE, "golden", "tile_tasks.json")))
+
+
+def task_bytes(entry):
+    return zlib.decompress(base64.b64decode(entry["bytes"]))
+
+
+def tasks(accepted=None):
+    return [t for t in FIXTURE["tasks"] if accepted is None or t["accepted"] == accepted]
+
+
+def test_wire_sizes_match_the_reference_layout():
+    lay = FIXTURE["layout"]
+    assert lay["task_bytes"] == rtsim.RT_TILE_TASK_BYTES == 11080
+    assert lay["header_bytes"] == 16 and lay["push_offset"] == 24 and lay["push_bytes"] == 11056
+    assert rtsim.RT_TILE_EDGE == 32
+    assert (rtsim.RT_TILE_MAX_SCREEN_WIDTH, rtsim.RT_TILE_MAX_SCREEN_HEIGHT) == (7680, 4320)
+
+
+def test_tile_geometry_matches_reference():
+    for case in FIXTURE["geometry"]:
+        if case["ok"]:
+            t = rtsim.tile_geometry(case["index"], case["width"], case["height"])
+            assert [t.origin_x, t.origin_y, t.width, t.height] == case["tile"], case
+        else:
+            with pytest.raises(rtsim.RtError):
+                rtsim.tile_geometry(case["index"], case["width"], case["height"])
+
+
+def test_tile_geometry_known_answers():
+    """The tiles the reference's tests use, and the ragged last row/column."""
+    t = rtsim.tile_geometry(78, 384, 384)  # test_cluster_worker_execution.cpp:70
+    assert (t.origin_x, t.origin_y, t.width, t.height) == (192, 192, 32, 32)
+    t = rtsim.tile_geometry(42, 1920, 1080)  # test_cluster_serialization_codecs.cpp:194
+    assert (t.origin_x, t.origin_y, t.width, t.height) == (1344, 0, 32, 32)
+    t = rtsim.tile_geometry(60 * 34 - 1, 1920, 1080)
+    assert (t.origin_x, t.origin_y, t.width, t.height) == (1888, 1056, 32, 24)
+    # the tiles of a frame cover it exactly once
+    W, H = 100, 70
+    cover = np.zeros((H, W), int)
+    for i in range(4 * 3):
+        t = rtsim.tile_geometry(i, W, H)
+        cover[t.origin_y:t.origin_y + t.height, t.origin_x:t.origin_x + t.width] += 1
+    assert np.all(cover == 1)
+
+
+def check_decoded(t, e):
+    assert t.tile_index == e["tile_index"]
+    assert [t.tile.origin_x, t.tile.origin_y, t.tile.width, t.tile.height] == e["tile"]
+    assert (t.lod_reduced_steps, t.flags, t.compression_type) == (e["lod_reduced_steps"], e["flags"],
+                                                                  e["compression_type"])
+    assert t.dispatch_timestamp == e["dispatch_timestamp"]
+    assert (t.metric_type, t.body_count) == (e["metric_type"], e["body_count"])
+    c = t.camera
+    assert list(c.observer) == e["observer"]
+    assert list(c.forward) == e["forward"] and list(c.right) == e["right"] and list(c.up) == e["up"]
+    assert c.fov_y == e["fov_y"] and (c.width, c.height) == (e["width"], e["height"])
+    assert (c.row_offset, c.row_count) == (0, 0)
+    assert t.metric.n_sources == 0 and t.metric.mass == e["mass"] and t.metric.spin == e["spin"]
+    assert t.integrator.max_steps == e["max_steps"]
+    d = rtsim.default_integrator()
+    assert (t.integrator.rtol, t.integrator.atol, t.integrator.h0) == (d.rtol, d.atol, d.h0)
+    tm = t.termination
+    assert tm.escape_radius == e["escape_radius"] and tm.disk_enabled == e["disk_enabled"]
+    assert tm.horizon_eps == 1e-6 and tm.flags == 0
+    assert tm.disk_inner == pytest.approx(e["disk_inner"], rel=1e-14, abs=0.0)
+    assert tm.disk_outer == pytest.approx(e["disk_outer"], rel=1e-14)
+
+
+@pytest.mark.parametrize("entry", tasks(), ids=lambda e: e["name"])
+def test_decode_matches_reference_verdict_and_fields(entry):
+    data = task_bytes(entry)
+    if not entry["accepted"]:
+        with pytest.raises(rtsim.RtError):
+            rtsim.decode_tile_task(data)
+        return
+    check_decoded(rtsim.decode_tile_task(data), entry["expected"])
+
+
+def test_decode_ignores_trailing_bytes():
+    entry = next(t for t in tasks(True) if t["name"] == "kerr_centre_tile")
+    t = rtsim.decode_tile_task(task_bytes(entry) + b"\x00" * 100)
+    check_decoded(t, entry["expected"])
+
+
+def test_metric_family_refusals_are_the_only_extra_rule():
+    extra = [t["name"] for t in FIXTURE["tasks"] if t["reference_accepts"] and not t["accepted"]]
+    assert sorted(extra) == ["metric_flat", "metric_reissner_nordstrom"]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+def test_random_mutations_match_reference_verdict():
+    """Random byte changes over the header, the timestamp and the first 800
+    bytes of the push constants: the decoder accepts exactly what the
+    reference's deserialize_render_tile_task accepts, except a metric_type
+    outside 1..3."""
+    ref = O.ref()
+    ref.ref_accepts_tile_task.argtypes = [C.c_void_p, C.c_size_t]
+    ref.ref_accepts_tile_task.restype = C.c_int
+    base = task_bytes(next(t for t in tasks(True) if t["name"] == "kerr_centre_tile"))
+    push = FIXTURE["layout"]["push_offset"]
+    mt_off = push + FIXTURE["layout"]["metric_type"]
+    rng = np.random.default_rng(7)
+    n_refused = 0
+    for _ in range(3000):
+        data = bytearray(base)
+        for _ in range(int(rng.integers(1, 4))):
+            off = int(rng.integers(0, push + 800))
+            data[off] = int(rng.integers(0, 256))
+        data = bytes(data)
+        want = bool(ref.ref_accepts_tile_task(data, len(data)))
+        metric_type = int.from_bytes(data[mt_off:mt_off + 4], "little")
+        want = want and 1 <= metric_type <= 3
+        try:
+            rtsim.decode_tile_task(data)
+            got = True
+        except rtsim.RtError:
+            got = False
+        n_refused += not got
+        assert got == want, data[:32].hex()
+    assert n_refused > 100
