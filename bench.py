@@ -8,6 +8,7 @@ generated on the device and results stay in HBM: the four frames are one
 batch (rt_trace_batch, one persistent kernel, replayed from a hipGraph);
 --path plain traces them with one launch per frame (rt_trace_camera_resident).
 --workload uneven512 times the 512-request batched workload instead.
+The "tiles" object reports the 32x32 tile-task path on the a = 0.9 frame.
 
 Prints one JSON line. value = rays/s over the timed steps. roofline: the
 algorithmic FP64 FLOP per launch (right-hand-side evaluations x FLOP_RHS +
@@ -199,6 +200,59 @@ def precision_report(side: int, switch_radius: float):
     return rep
 
 
+def tile_report(side: int = SIDE, spin: float = 0.9, n_sample: int = 64, seed: int = 7):
+    """The 32x32 tile path on one bench frame (a = 0.9): every tile of the
+    frame as one rt_trace_tiles batch (device time and rays/s, beside the
+    same frame as one rt_trace_camera launch), and single-tile rt_trace_tile
+    latency (device and host time) over the tiles around the hole's image
+    plus a random sample of the rest."""
+    cam = rtsim.default_camera(R_OBS, math.radians(INCL_DEG), 0.0, math.radians(FOV_DEG), side, side)
+    met, ig, tm = rtsim.kerr_metric(1.0, spin), rtsim.default_integrator(), rtsim.default_termination()
+    n_tiles = ((side + 31) // 32) ** 2
+    tasks = []
+    for i in range(n_tiles):
+        t = rtsim.rt_tile_task()
+        t.tile_index, t.camera, t.metric, t.integrator, t.termination = i, cam, met, ig, tm
+        t.tile = rtsim.tile_geometry(i, side, side)
+        tasks.append(t)
+    ctx = rtsim.Context(0)
+    ctx.trace_camera_resident(cam, met, ig, tm)
+    frame_ms = []
+    for _ in range(3):
+        ctx.trace_camera_resident(cam, met, ig, tm)
+        frame_ms.append(ctx.stats().kernel_ms)
+    arr = (rtsim.rt_tile_task * n_tiles)(*tasks)
+    ctx.trace_tiles(arr)  # captures the batch graph
+    batch_ms, wall = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        n = ctx.trace_tiles(arr)
+        wall.append(time.perf_counter() - t0)
+        batch_ms.append(ctx.stats().kernel_ms)
+    cols = (side + 31) // 32
+    c = cols // 2
+    centre = [(c + dy) * cols + (c + dx) for dy in range(-2, 2) for dx in range(-2, 2)]
+    rng = np.random.default_rng(seed)
+    rest = [int(i) for i in rng.choice(n_tiles, size=n_sample - len(centre), replace=False) if i not in centre]
+    dev, host = [], []
+    for i in centre + rest:
+        t = tasks[i]
+        t0 = time.perf_counter()
+        ctx.trace_tile(t.camera, t.metric, t.integrator, t.termination, t.tile)
+        host.append((time.perf_counter() - t0) * 1e3)
+        dev.append(ctx.stats().kernel_ms)
+    ctx.close()
+
+    def pct(v):
+        v = np.sort(np.asarray(v))
+        return {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99)), "max": float(v[-1])}
+
+    return {"frame": f"kerr_{side}sq_a{spin}_i60_robs1000_fov20", "tiles": n_tiles,
+            "batch_kernel_ms": float(np.median(batch_ms)), "batch_rays_per_s": n / float(np.median(wall)),
+            "frame_kernel_ms": float(np.median(frame_ms)), "single_tile_sampled": len(dev),
+            "single_tile_centre": len(centre), "single_tile_kernel_ms": pct(dev), "single_tile_host_ms": pct(host)}
+
+
 def copy_probe(mib: int):
     """Three plain device-to-device copies of a known size: what FETCH_SIZE
     and WRITE_SIZE read for them calibrates the counters against bytes."""
@@ -229,6 +283,7 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=50.0)
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tiles", action="store_true", help="skip the 32x32 tile-task report")
     ap.add_argument("--workload", choices=["frames4", "uneven512"], default="frames4",
                     help="frames4: the four 2048^2 frames; uneven512: 512 seeded camera requests in one batch")
     ap.add_argument("--precision-report", type=int, default=0, metavar="SIDE",
@@ -334,6 +389,9 @@ def main() -> int:
     if not args.no_serve:
         print(f"bench: serving {args.serve_requests} requests at {args.qps} QPS", file=sys.stderr, flush=True)
         out["latency"] = serve_latency(args.qps, args.serve_requests)
+    if not args.no_tiles:
+        print("bench: 32x32 tile tasks", file=sys.stderr, flush=True)
+        out["tiles"] = tile_report(args.side)
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         print(f"bench: cpu baseline on {threads} threads", file=sys.stderr, flush=True)

@@ -24,6 +24,8 @@
  *                        (src/distributed/task_payloads.cpp:70-129)
  *   rt_trace_tile        SoftwareComputeEngine::dispatch_tile
  *                        (include/relativistic/render/software_compute_engine.hpp:905-1022)
+ *   rt_trace_tiles       many tile tasks in one dispatch: the task loop of
+ *                        TileRenderer::render (src/distributed/tile_renderer.cpp:133-182)
  *   rt_server_*          the windowed worker of GeodesicComputePipeline
  *                        (include/relativistic/render/geodesic_compute_pipeline.hpp:529-728):
  *                        requests, cancellation, completion polling
@@ -318,6 +320,22 @@ rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* o
 rt_status rt_trace_tile(rt_context* ctx, const rt_camera* cam, const rt_metric* metric,
                         const rt_integrator* integ, const rt_termination* term, const rt_tile* region,
                         size_t* n_rays);
+
+/* Trace n_tasks decoded tile tasks (any mix of frames, metrics and
+ * tolerances) as one batch: a persistent kernel over a shared work queue of
+ * 8x8 pixel blocks (queue_kind 0 at two waves per SIMD when the context's
+ * shape asks for per-request kernels; a tile is too small to fill the device
+ * alone),
+ * captured in a hipGraph that is replayed while the batch layout stays the
+ * same. Results stay device-resident, task after task, each tile-local
+ * row-major, for rt_fetch_results / rt_fetch_pixels; *n_rays receives their
+ * total. Each task's results are rt_trace_tile's for it to rounding (class,
+ * flags and step counts identical; another kernel, the same per-ray
+ * arithmetic).
+ * Replaces the per-task loop of a tile worker, TileRenderer::render called
+ * once per RenderTileTaskPayload (src/distributed/tile_renderer.cpp:133-182),
+ * whose GPU dispatches gpu_mutex_ serialises. */
+rt_status rt_trace_tiles(rt_context* ctx, const rt_tile_task* tasks, size_t n_tasks, size_t* n_rays);
 
 /* ------------------------------------------------------- launch shapes */
 typedef struct rt_launch_shape {

@@ -46,31 +46,49 @@ struct BatchPlan {
     std::vector<size_t> n_rays;          // per request
     bool mixed = false;   // some request uses the mixed precision policy
     bool carter = false;  // some request asks for RT_TERM_CARTER_CAPTURE
+    bool concat = false;  // results of all requests in one array, request after request
     size_t result_bytes = 0, table_bytes = 0;
 };
 
+// Is rectangle w non-empty and inside the camera's frame?
+bool valid_window(const rt_camera& c, const rt_tile& w, std::string& why) {
+    if (w.width == 0 || w.height == 0 || w.origin_x >= c.width || w.origin_y >= c.height ||
+        w.width > c.width - w.origin_x || w.height > c.height - w.origin_y) {
+        why = "region: empty, or not inside the camera's frame";
+        return false;
+    }
+    return true;
+}
+
 // Validate and lay out a batch (no device work). tiles: queue entries are
-// 8x8 tiles (k_trace_tiles) rather than rays (k_trace_queue).
-rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool tiles) {
+// 8x8 tiles (k_trace_tiles) rather than rays (k_trace_queue). wins (may be
+// null): request i traces rectangle wins[i] of its frame; concat: the results
+// of all requests share one array, request after request.
+rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool tiles, const rt_tile* wins = nullptr,
+                     bool concat = false) {
     plan = BatchPlan{};
     plan.n_rays.resize(n);
+    plan.concat = concat;
     std::vector<uint64_t> hot_len, cold_len;  // queue entries per request, hot and other tiles
+    size_t total = 0;
     for (size_t i = 0; i < n; ++i) {
         const rt_request& r = reqs[i];
+        const rt_tile* win = wins ? &wins[i] : nullptr;
         std::string why;
-        if (!valid_request(&r.metric, &r.integrator, &r.termination, why) || !valid_camera(&r.camera, why))
+        if (!valid_request(&r.metric, &r.integrator, &r.termination, why) || !valid_camera(&r.camera, why) ||
+            (win && !valid_window(r.camera, *win, why)))
             return fail(RT_ERR_INVALID_ARG, "request " + std::to_string(i) + ": " + why);
         TraceParams P;
         fill_params(&r.metric, &r.integrator, &r.termination, P);
         if (P.n_src >= 2) {
             MultiDev T;
             fill_sources(&r.metric, &r.termination, T);
-            fill_camera(&r.camera, P, &T);
+            fill_camera(&r.camera, P, &T, win);
             plan.m_req.push_back(static_cast<uint32_t>(i));
             plan.m_params.push_back(P);
             plan.m_tables.push_back(T);
         } else {
-            fill_camera(&r.camera, P, nullptr);
+            fill_camera(&r.camera, P, nullptr, win);
             plan.q_req.push_back(static_cast<uint32_t>(i));
             plan.q_params.push_back(P);
             const uint64_t tiles_y = (P.n_rays / P.cols + 7) / 8;
@@ -83,7 +101,12 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool til
             plan.carter = plan.carter || P.carter;
         }
         plan.n_rays[i] = P.n_rays;
-        plan.result_bytes += results_bytes(P.n_rays, false);
+        total += P.n_rays;
+        if (!concat) plan.result_bytes += results_bytes(P.n_rays, false);
+    }
+    if (concat) {
+        if (total > (1ull << 31)) return fail(RT_ERR_INVALID_ARG, "more than 2^31 rays in one batch");
+        plan.result_bytes = results_bytes(total, false);
     }
     // queue segments: every request's hot tiles first, then the other tiles
     plan.q_offset.assign(1, 0);
@@ -98,6 +121,7 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool til
 // Device pointers of a placed batch.
 struct Placed {
     std::vector<OutPtrs> outs;           // per request (index = request index)
+    OutPtrs all{};                       // concatenated batches: the whole array
     QueueDev Q{};
     unsigned long long* counters = nullptr;
     unsigned long long* heads = nullptr; // queue head of each superposition request
@@ -119,9 +143,32 @@ void place_batch(const BatchPlan& plan, char* results, char* tables, const Launc
     pl.heads = reinterpret_cast<unsigned long long*>(tb);
     tb += al256(nm * 8);
     pl.n_zero = (kCounterBytes + al256(nm * 8)) / 8;
-    for (size_t i = 0; i < n; ++i) {
-        pl.outs[i] = carve_results(cur, plan.n_rays[i], false);
-        pl.outs[i].counters = pl.counters;
+    if (plan.concat) {
+        // one array of all rays; request i's view starts at its first ray and
+        // keeps the whole array's leading dimension
+        size_t total = 0;
+        for (size_t v : plan.n_rays) total += v;
+        pl.all = carve_results(cur, total, false);
+        pl.all.counters = pl.counters;
+        size_t off = 0;
+        for (size_t i = 0; i < n; ++i) {
+            OutPtrs o = pl.all;
+            o.cls += off;
+            o.x += off;
+            o.p += off;
+            o.bl += off;
+            o.lambda += off;
+            o.g += off;
+            o.steps += off;
+            o.rejected += off;
+            pl.outs[i] = o;
+            off += plan.n_rays[i];
+        }
+    } else {
+        for (size_t i = 0; i < n; ++i) {
+            pl.outs[i] = carve_results(cur, plan.n_rays[i], false);
+            pl.outs[i].counters = pl.counters;
+        }
     }
     pl.d_params = reinterpret_cast<TraceParams*>(tb);
     tb += al256(nq * sizeof(TraceParams));
@@ -261,34 +308,42 @@ static GraphCache& graph_cache(const rt_context* ctx) {
 }
 
 // Run a batch on the context: place, upload, replay or capture the graph, time it.
-static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, rt_results* outs) {
+// wins (may be null): per-request rectangles, results concatenated in request
+// order and left resident as the context's last trace; when the context's
+// shape asks for per-request kernels they run on the shared tile queue at two
+// waves per SIMD instead (the serving loop's shape), a rectangle being too
+// small to fill the device alone.
+static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, rt_results* outs,
+                           const rt_tile* wins = nullptr) {
     const auto t0 = std::chrono::steady_clock::now();
     RT_HIP(hipSetDevice(ctx->device));
+    LaunchShape shape = ctx->qshape;
+    if (wins && shape.queue_kind == 2) shape = kServerShape;
     BatchPlan plan;
-    rt_status st = plan_batch(reqs, n_req, plan, ctx->qshape.queue_kind != 1);
+    rt_status st = plan_batch(reqs, n_req, plan, shape.queue_kind != 1, wins, wins != nullptr);
     if (st != RT_OK) return st;
     st = ctx->arena.ensure(std::max<size_t>(plan.result_bytes, 256));
     if (st != RT_OK) return st;
     st = ctx->aux.ensure(plan.table_bytes + sizeof(MultiDev) + 256);
     if (st != RT_OK) return st;
     Placed pl;
-    place_batch(plan, static_cast<char*>(ctx->arena.base), static_cast<char*>(ctx->aux.base), ctx->qshape, pl);
+    place_batch(plan, static_cast<char*>(ctx->arena.base), static_cast<char*>(ctx->aux.base), shape, pl);
     st = upload_batch(plan, pl, ctx->stream);
     if (st != RT_OK) return st;
     GraphCache& gc = graph_cache(ctx);
     (void)0;
-    std::vector<unsigned char> key = graph_key(plan, pl, ctx->qshape);
+    std::vector<unsigned char> key = graph_key(plan, pl, shape);
     if (!gc.exec || gc.key != key) {
         gc.reset();
         uint32_t grid = 0;
-        RT_HIP(queue_grid(ctx->qshape, plan.mixed, plan.carter, &grid));
+        RT_HIP(queue_grid(shape, plan.mixed, plan.carter, &grid));
         RT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         Branches br;
         br.side = ctx->side;
         br.fork = ctx->fork;
         br.join = ctx->join;
         br.n = rt_context::kSide;
-        const hipError_t le = launch_batch(plan, pl, ctx->qshape, grid, ctx->stream, br);
+        const hipError_t le = launch_batch(plan, pl, shape, grid, ctx->stream, br);
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
         if (le != hipSuccess) {
@@ -315,8 +370,13 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
             st = copy_results_to_host(pl.outs[i], plan.n_rays[i], &outs[i], ctx->stream);
             if (st != RT_OK) return st;
         }
-    ctx->last = n_req ? pl.outs[0] : OutPtrs{};
-    ctx->n_last = n_req ? plan.n_rays[0] : 0;
+    if (plan.concat) {
+        ctx->last = pl.all;
+        ctx->n_last = total;
+    } else {
+        ctx->last = n_req ? pl.outs[0] : OutPtrs{};
+        ctx->n_last = n_req ? plan.n_rays[0] : 0;
+    }
     rt_stats& s = ctx->stats;
     s.kernel_ms = ms;
     s.rays = total;
@@ -324,9 +384,9 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
     s.rejected_steps = cnt[1];
     s.rhs_evals = cnt[2];
     s.fp32_steps = cnt[3];
-    s.block_size = ctx->qshape.block;
+    s.block_size = shape.block;
     uint32_t grid = 0;
-    (void)queue_grid(ctx->qshape, plan.mixed, plan.carter, &grid);
+    (void)queue_grid(shape, plan.mixed, plan.carter, &grid);
     s.grid_size = grid;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RT_OK;
@@ -609,6 +669,24 @@ rt_status rt_trace_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, 
     if (!ctx || !reqs || n_req == 0) return fail(RT_ERR_INVALID_ARG, "null context/requests or n_req == 0");
     std::lock_guard<std::mutex> lk(ctx->mu);
     return run_batch(ctx, reqs, n_req, outs);
+}
+
+rt_status rt_trace_tiles(rt_context* ctx, const rt_tile_task* tasks, size_t n_tasks, size_t* n_rays) {
+    if (!ctx || !tasks || n_tasks == 0) return fail(RT_ERR_INVALID_ARG, "null context or tasks, or no tasks");
+    std::vector<rt_request> reqs(n_tasks);
+    std::vector<rt_tile> wins(n_tasks);
+    for (size_t i = 0; i < n_tasks; ++i) {
+        reqs[i].camera = tasks[i].camera;
+        reqs[i].metric = tasks[i].metric;
+        reqs[i].integrator = tasks[i].integrator;
+        reqs[i].termination = tasks[i].termination;
+        wins[i] = tasks[i].tile;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    rt_status st = run_batch(ctx, reqs.data(), n_tasks, nullptr, wins.data());
+    if (st != RT_OK) return st;
+    if (n_rays) *n_rays = ctx->n_last;
+    return RT_OK;
 }
 
 rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample, rt_launch_shape* best) {

@@ -257,7 +257,7 @@ EXPORTS = [
     "rt_default_integrator", "rt_default_termination", "rt_default_camera", "rt_kerr_metric",
     "rt_abi_version", "rt_version", "rt_last_error", "rt_context_create", "rt_context_destroy",
     "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
-    "rt_fetch_pixels", "rt_tile_geometry", "rt_decode_tile_task", "rt_trace_tile",
+    "rt_fetch_pixels", "rt_tile_geometry", "rt_decode_tile_task", "rt_trace_tile", "rt_trace_tiles",
     "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
     "rt_server_wait", "rt_server_take", "rt_server_cancel",
 ]
@@ -291,6 +291,7 @@ def load(path: Optional[str] = None) -> C.CDLL:
     lib.rt_decode_tile_task.argtypes = [C.c_void_p, C.c_size_t, P(rt_tile_task)]
     lib.rt_trace_tile.argtypes = [C.c_void_p, P(rt_camera), P(rt_metric), P(rt_integrator), P(rt_termination),
                                   P(rt_tile), P(C.c_size_t)]
+    lib.rt_trace_tiles.argtypes = [C.c_void_p, P(rt_tile_task), C.c_size_t, P(C.c_size_t)]
     lib.rt_trace_batch.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_results)]
     lib.rt_get_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
     lib.rt_set_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
@@ -382,6 +383,15 @@ class Context:
         n = C.c_size_t()
         _check(self.lib, self.lib.rt_trace_tile(self.h, C.byref(cam), C.byref(metric), C.byref(integ),
                                                 C.byref(term), C.byref(tile), C.byref(n)))
+        return n.value
+
+    def trace_tiles(self, tasks) -> int:
+        """Trace decoded tile tasks (a sequence, or a ctypes array of
+        rt_tile_task) as one batch; results stay resident, task after task
+        (each tile-local row-major). Returns the total ray count."""
+        arr = tasks if isinstance(tasks, C.Array) else (rt_tile_task * len(tasks))(*tasks)
+        n = C.c_size_t()
+        _check(self.lib, self.lib.rt_trace_tiles(self.h, arr, len(tasks), C.byref(n)))
         return n.value
 
     def fetch(self, n: int) -> Results:

@@ -9,6 +9,7 @@ import pytest
 
 import oracle_lib as O
 import rtsim
+from test_gpu_batch import same
 from test_gpu_multi import compare as compare_multi
 from test_gpu_parity import compare
 from test_tile_task import task_bytes, tasks
@@ -94,3 +95,74 @@ def test_refused_regions(ctx):
     t = rtsim.rt_tile(8, 4, 32, 26)
     n = ctx.trace_tile(cam, met, ig, tm, t)
     compare(ctx.fetch(n), oracle_window(cam, met, ig, tm, t), "after refusals")
+
+
+def fixture_task(name):
+    return rtsim.decode_tile_task(task_bytes(next(e for e in tasks(True) if e["name"] == name)))
+
+
+def split(res, sizes):
+    out, off = [], 0
+    for n in sizes:
+        idx = np.arange(off, off + n)
+        sub = rtsim.Results(n)
+        sub.cls[:], sub.lam[:], sub.g[:] = res.cls[idx], res.lam[idx], res.g[idx]
+        sub.steps[:], sub.rejected[:] = res.steps[idx], res.rejected[idx]
+        sub.x[:], sub.p[:], sub.bl[:] = res.x[:, idx], res.p[:, idx], res.bl[:, idx]
+        out.append(sub)
+        off += n
+    return out
+
+
+def test_tile_batch_equals_single_tiles(ctx):
+    """A batch of tasks of different frames, metrics (one a superposition),
+    tolerances and tile shapes: each task's slice of the batch equals
+    rt_trace_tile of it (class, flags and step counts identical, coordinates
+    within 1e-12: the shared tile-queue kernel runs the same per-ray
+    arithmetic), on two replays and after a smaller batch."""
+    names = ["kerr_centre_tile", "ragged_corner_tile", "worker_test_task", "scaled_disk_tile", "disk_off_tile"]
+    batch = [fixture_task(n) for n in names]
+    multi = fixture_task("kerr_centre_tile")
+    multi.metric = rtsim.multi_metric([{"mass": 1.0, "spin": 0.9},
+                                       {"mass": 0.3, "spin": 0.5, "position": (12.0, -4.0, 2.0)}])
+    batch.append(multi)
+    loose = fixture_task("kerr_centre_tile")
+    loose.integrator.rtol = 1e-8
+    batch.append(loose)
+    singles = []
+    for t in batch:
+        n = ctx.trace_tile(t.camera, t.metric, t.integrator, t.termination, t.tile)
+        singles.append(ctx.fetch(n))
+    sizes = [t.tile.width * t.tile.height for t in batch]
+    for rep in range(2):
+        print(f"[tile batch] {len(batch)} tasks, replay {rep}", flush=True)
+        n = ctx.trace_tiles(batch)
+        assert n == sum(sizes)
+        for i, part in enumerate(split(ctx.fetch(n), sizes)):
+            same(part, singles[i], f"task {i} replay {rep}")
+    small = batch[1:3]
+    n = ctx.trace_tiles(small)
+    for i, part in enumerate(split(ctx.fetch(n), sizes[1:3])):
+        same(part, singles[1 + i], f"smaller batch task {i}")
+    pix = ctx.fetch_pixels(n)
+    assert np.array_equal(pix["iterations_used"], np.concatenate([s.steps for s in singles[1:3]]))
+
+
+def test_all_tiles_of_a_frame(ctx):
+    """The 64 tiles of the 256^2 CPU-runnable configuration, as one batch,
+    reassemble the frame rt_trace_camera traces (to the same 1e-12)."""
+    base = fixture_task("kerr_centre_tile")
+    cam, W, H = base.camera, base.camera.width, base.camera.height
+    batch = []
+    for i in range(((W + 31) // 32) * ((H + 31) // 32)):
+        t = rtsim.rt_tile_task.from_buffer_copy(base)
+        t.tile_index = i
+        t.tile = rtsim.tile_geometry(i, W, H)
+        batch.append(t)
+    n = ctx.trace_tiles(batch)
+    assert n == W * H
+    res = ctx.fetch(n)
+    frame = ctx.trace_camera(cam, base.metric, base.integrator, base.termination)
+    parts = split(res, [t.tile.width * t.tile.height for t in batch])
+    for t, part in zip(batch, parts):
+        same(part, window(frame, W, t.tile), f"tile {t.tile_index}")
