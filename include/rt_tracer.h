@@ -337,6 +337,28 @@ rt_status rt_trace_tile(rt_context* ctx, const rt_camera* cam, const rt_metric* 
  * whose GPU dispatches gpu_mutex_ serialises. */
 rt_status rt_trace_tiles(rt_context* ctx, const rt_tile_task* tasks, size_t n_tasks, size_t* n_rays);
 
+/* ------------------------------------------- allocation-free tile batch
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, serving loops that must not call
+ * hipMalloc, harnesses that forbid it. Traces n_tasks decoded tile tasks
+ * (single Kerr) like rt_trace_tiles, on the same work-queue kernel at two
+ * waves per SIMD, but allocates nothing: the per-task tables, counters and
+ * queue head live in a fixed device array of the library (64 KiB per GPU,
+ * held until the process ends), and the results go to the caller's device
+ * arrays. dev_out holds DEVICE pointers, every one non-NULL and naturally
+ * aligned, each array sized for `capacity` rays (x and p: 4 * capacity,
+ * bl: 3 * capacity, laid out component after component with leading
+ * dimension capacity). Results are written task after task, each tile-local
+ * row-major; *n_rays receives their total. RT_ERR_INVALID_ARG for more than
+ * RT_NOALLOC_MAX_TASKS tasks, for more rays than capacity, and for a
+ * superposition task. Calls on one GPU share its array: the library
+ * serialises them; work launched by other means that overlaps a call on
+ * the same GPU and uses that array would race. Returns after the kernel
+ * finished; the context's resident results are left as they were. */
+#define RT_NOALLOC_MAX_TASKS 64u
+rt_status rt_trace_tiles_noalloc(rt_context* ctx, const rt_tile_task* tasks, size_t n_tasks,
+                                 const rt_results* dev_out, size_t capacity, size_t* n_rays);
+
 /* ------------------------------------------------------- launch shapes */
 typedef struct rt_launch_shape {
     uint32_t block;       /* threads per workgroup: 64, 128, 192 or 256          */

@@ -636,6 +636,11 @@ rt_status rt_context_create(int device, rt_context** out) {
         rt_context_destroy(ctx);
         return hip_fail(e, "context set-up");
     }
+    const rt_status ss = rt::scratch_setup(device);
+    if (ss != RT_OK) {
+        rt_context_destroy(ctx);
+        return ss;
+    }
     *out = ctx;
     return RT_OK;
 }

@@ -52,6 +52,10 @@ size_t results_bytes(size_t n, bool with_inputs);
 
 rt_status copy_results_to_host(const OutPtrs& d, size_t n, const rt_results* out, hipStream_t s);
 
+// Device address of this device's allocation-free scratch array, taken once
+// per device with the device current (rt_noalloc.hip).
+rt_status scratch_setup(int device);
+
 struct GraphCache;
 // Drop the captured batch graph of a context (on destroy).
 void forget_graphs(const rt_context* ctx);

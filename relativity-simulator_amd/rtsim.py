@@ -99,6 +99,7 @@ RT_TILE_EDGE = 32
 RT_TILE_MAX_SCREEN_WIDTH = 7680
 RT_TILE_MAX_SCREEN_HEIGHT = 4320
 RT_TILE_TASK_BYTES = 11080
+RT_NOALLOC_MAX_TASKS = 64
 
 
 class rt_tile(C.Structure):
@@ -259,7 +260,7 @@ EXPORTS = [
     "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
     "rt_fetch_pixels", "rt_tile_geometry", "rt_decode_tile_task", "rt_trace_tile", "rt_trace_tiles",
     "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
-    "rt_server_wait", "rt_server_take", "rt_server_cancel",
+    "rt_server_wait", "rt_server_take", "rt_server_cancel", "rt_trace_tiles_noalloc",
 ]
 
 
@@ -292,6 +293,8 @@ def load(path: Optional[str] = None) -> C.CDLL:
     lib.rt_trace_tile.argtypes = [C.c_void_p, P(rt_camera), P(rt_metric), P(rt_integrator), P(rt_termination),
                                   P(rt_tile), P(C.c_size_t)]
     lib.rt_trace_tiles.argtypes = [C.c_void_p, P(rt_tile_task), C.c_size_t, P(C.c_size_t)]
+    lib.rt_trace_tiles_noalloc.argtypes = [C.c_void_p, P(rt_tile_task), C.c_size_t, P(rt_results), C.c_size_t,
+                                           P(C.c_size_t)]
     lib.rt_trace_batch.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_results)]
     lib.rt_get_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
     lib.rt_set_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
