@@ -35,6 +35,7 @@ sys.path.insert(0, os.path.join(REPO, "relativity-simulator_amd"))
 import numpy as np  # noqa: E402
 
 import rtsim  # noqa: E402
+import workloads  # noqa: E402
 
 SPINS = (0.0, 0.5, 0.9, 0.998)
 SIDE = 2048
@@ -58,12 +59,7 @@ METRIC = "rays/s Kerr 2048\u00b2 mixed vs FP64 at \u22641e-9 rel err; p50/p99 la
 
 
 def frame_requests(side: int, precision: int):
-    reqs = []
-    for a in SPINS:
-        cam = rtsim.default_camera(R_OBS, math.radians(INCL_DEG), 0.0, math.radians(FOV_DEG), side, side)
-        reqs.append((cam, rtsim.kerr_metric(1.0, a), rtsim.default_integrator(precision=precision),
-                     rtsim.default_termination()))
-    return reqs
+    return workloads.frame_requests(side, precision, INCL_DEG)
 
 
 def cpu_baseline(side: int, stride: int, threads: int):
@@ -86,39 +82,11 @@ def cpu_baseline(side: int, stride: int, threads: int):
                       f"oracle/{os.path.basename(oracle_lib.ORACLE_PATH)}"}
 
 
-def serving_requests(n: int, seed: int):
-    """Trace requests of the serving workload: lognormal ray counts (median
-    256^2, sigma 1, at most 2048^2, square frames), 70% single Kerr with
-    a ~ U[0, 0.998], 30% superpositions of 2..16 holes (masses U[0.1, 1],
-    positions U[-25, 25]^3 M, spins U[0, 0.95] on random axes); observer at
-    r = 1000 M, inclination U[5, 89] deg, FOV 20 deg, rtol 1e-10."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        rays = min(2048 * 2048, int(round(65536 * math.exp(rng.normal()))))
-        side = max(8, int(round(math.sqrt(rays))))
-        cam = rtsim.default_camera(R_OBS, math.radians(rng.uniform(5, 89)), rng.uniform(-math.pi, math.pi),
-                                   math.radians(FOV_DEG), side, side)
-        if rng.random() < 0.3:
-            k = int(rng.integers(2, 17))
-            src = [{"mass": 1.0, "spin": float(rng.uniform(0, 0.95))}]
-            for _ in range(k - 1):
-                src.append({"mass": float(rng.uniform(0.1, 1.0)), "spin": float(rng.uniform(0, 0.95)),
-                            "position": tuple(float(v) for v in rng.uniform(-25, 25, size=3)),
-                            "axis": tuple(float(v) for v in rng.normal(size=3))})
-            met = rtsim.multi_metric(src)
-        else:
-            met = rtsim.kerr_metric(1.0, float(rng.uniform(0, 0.998)))
-        out.append(rtsim.make_request(cam, met, rtsim.default_integrator(), rtsim.default_termination()))
-    return out
-
-
 def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
     """Poisson arrivals at a fixed QPS into the serving loop; latency is
     submit -> results on the host. Reported overall and per request kind."""
-    reqs = serving_requests(n, seed)
-    rng = np.random.default_rng(seed + 1)
-    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
+    reqs = workloads.serving_requests(n, seed)
+    arrivals = workloads.poisson_arrivals(n, qps, seed + 1)
     lat, kind = [], []
     rays = 0
     with rtsim.Server(0, hbm_budget_bytes=16 << 30, max_inflight=256) as srv:
@@ -150,22 +118,6 @@ def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
             "rays_per_s": rays / wall, "wall_s": wall, "multi_source_fraction": 0.3,
             "workload": "Poisson arrivals, lognormal rays (median 256^2, sigma 1, max 2048^2), "
                         "70% Kerr / 30% 2-16 hole superpositions"}
-
-
-def uneven_requests(n: int = 512, seed: int = 0xC0FFEE1234):
-    """Batched workload: sides U[64, 1024], a U[0, 0.998], inclination
-    U[5, 89] deg, r_obs log-uniform in [50, 5000] M, rtol in {1e-8, 1e-10}."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        side = int(rng.integers(64, 1025))
-        r_obs = float(math.exp(rng.uniform(math.log(50.0), math.log(5000.0))))
-        cam = rtsim.default_camera(r_obs, math.radians(rng.uniform(5, 89)), 0.0, math.radians(FOV_DEG), side, side)
-        rtol = 1e-8 if rng.random() < 0.5 else 1e-10
-        out.append(rtsim.make_request(cam, rtsim.kerr_metric(1.0, float(rng.uniform(0, 0.998))),
-                                      rtsim.default_integrator(rtol=rtol),
-                                      rtsim.default_termination(escape_radius=max(1000.0, r_obs))))
-    return out
 
 
 def precision_report(side: int, switch_radius: float):
@@ -305,7 +257,7 @@ def main() -> int:
     prec = rtsim.RT_PRECISION_MIXED if args.precision == "mixed" else rtsim.RT_PRECISION_FP64
     reqs = frame_requests(args.side, prec)
     if args.workload == "uneven512":
-        reqs = [(r.camera, r.metric, r.integrator, r.termination) for r in uneven_requests()]
+        reqs = [(r.camera, r.metric, r.integrator, r.termination) for r in workloads.uneven_requests()]
     ctx = rtsim.Context(0)
     tuned = None
     if args.workload == "uneven512" and args.path == "batch":

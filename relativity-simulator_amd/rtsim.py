@@ -421,15 +421,18 @@ class Context:
                                                 C.byref(integ), C.byref(term), C.byref(res.struct)))
         return res
 
-    def trace_batch(self, reqs: Sequence[rt_request], fetch: bool = True) -> Optional[List[Results]]:
+    def trace_batch(self, reqs: Sequence[rt_request], fetch: bool = True,
+                    only: Optional[Sequence[int]] = None) -> Optional[List[Optional[Results]]]:
         """All requests in one work-queue kernel (captured hipGraph, replayed
-        while the layout is unchanged). fetch=False keeps results on the device."""
+        while the layout is unchanged). fetch=False keeps results on the device;
+        only: copy back just these requests (the others' entries are None)."""
         arr = (rt_request * len(reqs))(*reqs)
         if not fetch:
             _check(self.lib, self.lib.rt_trace_batch(self.h, arr, len(reqs), None))
             return None
-        outs = [Results(n_rays_of(r.camera)) for r in reqs]
-        oarr = (rt_results * len(reqs))(*[o.struct for o in outs])
+        keep = set(range(len(reqs))) if only is None else set(only)
+        outs = [Results(n_rays_of(r.camera)) if i in keep else None for i, r in enumerate(reqs)]
+        oarr = (rt_results * len(reqs))(*[o.struct if o is not None else rt_results() for o in outs])
         _check(self.lib, self.lib.rt_trace_batch(self.h, arr, len(reqs), oarr))
         return outs
 

@@ -11,6 +11,7 @@ import math
 import numpy as np
 import pytest
 
+import oracle_lib as O
 import rtsim
 
 pytestmark = pytest.mark.gpu
@@ -87,3 +88,25 @@ def test_launch_shape_roundtrip(ctx):
     with pytest.raises(rtsim.RtError):
         ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(64, 1, 16, 3))
     ctx.set_launch_shapes(plain=plain, queue=queue)
+
+
+@pytest.mark.parametrize("kind", [0, 1], ids=["tile_queue", "lane_refill_queue"])
+def test_queue_kernels_vs_oracle(ctx, kind):
+    """The persistent queue kernels' results against the oracle directly, at
+    the FP64 parity bar (not only against the single-request kernel)."""
+    from test_gpu_multi import compare as compare_multi
+    from test_gpu_parity import compare
+
+    _, keep = ctx.launch_shapes()
+    ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(64, 2, 16, kind))
+    try:
+        reqs = uneven_requests()
+        outs = ctx.trace_batch(reqs)
+    finally:
+        ctx.set_launch_shapes(queue=keep)
+    for i, (r, o) in enumerate(zip(reqs, outs)):
+        ref = O.trace_camera(r.camera, r.metric, r.integrator, r.termination)
+        if r.metric.n_sources >= 2:
+            compare_multi(o, ref, f"kind {kind} request {i} (superposition)")
+        else:
+            compare(o, ref, f"kind {kind} request {i}")

@@ -1,7 +1,7 @@
 """Superposition (2..16 Kerr-Schild sources) on the GPU vs the CPU oracle.
 
-Bar: class identical; final KS position, BL (r, theta), affine parameter and
-g within 1e-9 relative (max(|ref|, 1) denominator) for rays that are not
+Bar: class identical; final KS position, BL (r, theta, phi; phi as a wrapped
+difference), affine parameter and g within 1e-9 relative (max(|ref|, 1) denominator) for rays that are not
 captured; captured rays compare class only (they end on a horizon)."""
 import math
 
@@ -17,6 +17,11 @@ TOL = 1e-9
 
 def rel(a, b):
     return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+
+
+def phi_err(a, b):
+    d = np.remainder(a - b + np.pi, 2.0 * np.pi) - np.pi
+    return np.abs(d) / np.maximum(np.abs(b), 1.0)
 
 
 def two_holes():
@@ -44,6 +49,7 @@ def compare(gpu, ref, label):
     errs = {"x": rel(gpu.x[1:, nc], ref.x[1:, nc]).max(initial=0),
             "r": rel(gpu.bl[0, nc], ref.bl[0, nc]).max(initial=0),
             "theta": rel(gpu.bl[1, nc], ref.bl[1, nc]).max(initial=0),
+            "phi": phi_err(gpu.bl[2, nc], ref.bl[2, nc]).max(initial=0),
             "lambda": rel(gpu.lam[nc], ref.lam[nc]).max(initial=0),
             "g": rel(gpu.g, ref.g).max(initial=0)}
     print(f"[{label}] max rel err " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)

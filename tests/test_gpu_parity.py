@@ -1,8 +1,8 @@
 """FP64 HIP tracer vs the CPU oracle on the same rays (through the C ABI).
 
 Bar: termination class identical; final KS position and BL (r, theta) within
-1e-9 relative (max(|ref|, 1) denominator), affine parameter within 1e-9
-relative; accepted step counts identical for >= 99% of rays and within 2
+1e-9 relative (max(|ref|, 1) denominator), BL phi within 1e-9 (wrapped
+difference, same denominator), affine parameter within 1e-9 relative; accepted step counts identical for >= 99% of rays and within 2
 otherwise. Captured rays end on r = r_+ + eps after following the horizon
 asymptotically, so only their class, r and theta are compared."""
 import math
@@ -22,6 +22,12 @@ def rel(a, b):
     return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
 
 
+def phi_err(a, b):
+    """Difference of two azimuths wrapped into [-pi, pi), relative to max(|b|, 1)."""
+    d = np.remainder(a - b + np.pi, 2.0 * np.pi) - np.pi
+    return np.abs(d) / np.maximum(np.abs(b), 1.0)
+
+
 def compare(gpu, ref, label):
     print(f"[{label}] n={ref.n}", flush=True)
     assert np.array_equal(gpu.klass, ref.klass), f"{label}: class mismatch at {np.where(gpu.klass != ref.klass)[0][:10]}"
@@ -31,6 +37,7 @@ def compare(gpu, ref, label):
         "x": rel(gpu.x[1:, nc], ref.x[1:, nc]).max(initial=0),
         "r": rel(gpu.bl[0], ref.bl[0]).max(initial=0),
         "theta": rel(gpu.bl[1], ref.bl[1]).max(initial=0),
+        "phi": phi_err(gpu.bl[2, nc], ref.bl[2, nc]).max(initial=0),
         "lambda": rel(gpu.lam[nc], ref.lam[nc]).max(initial=0),
         "g": rel(gpu.g, ref.g).max(initial=0),
         "p": rel(gpu.p[1:, nc], ref.p[1:, nc]).max(initial=0),

@@ -114,7 +114,19 @@ def split(res, sizes):
     return out
 
 
-def test_tile_batch_equals_single_tiles(ctx):
+@pytest.fixture(params=[0, 1, 2], ids=["tile_queue", "lane_refill_queue", "per_request_kind"])
+def batch_kind(ctx, request):
+    """The context's batch shape for one test, restored afterwards. Kind 2
+    (per-request kernels) makes rt_trace_tiles use the shared tile queue at
+    two waves per SIMD; kind 1 runs the lane-refill kernel over the
+    concatenated windows."""
+    _, keep = ctx.launch_shapes()
+    ctx.set_launch_shapes(queue=rtsim.rt_launch_shape(64, 2, 16, request.param))
+    yield request.param
+    ctx.set_launch_shapes(queue=keep)
+
+
+def test_tile_batch_equals_single_tiles(ctx, batch_kind):
     """A batch of tasks of different frames, metrics (one a superposition),
     tolerances and tile shapes: each task's slice of the batch equals
     rt_trace_tile of it (class, flags and step counts identical, coordinates
@@ -135,7 +147,7 @@ def test_tile_batch_equals_single_tiles(ctx):
         singles.append(ctx.fetch(n))
     sizes = [t.tile.width * t.tile.height for t in batch]
     for rep in range(2):
-        print(f"[tile batch] {len(batch)} tasks, replay {rep}", flush=True)
+        print(f"[tile batch] kind {batch_kind}: {len(batch)} tasks, replay {rep}", flush=True)
         n = ctx.trace_tiles(batch)
         assert n == sum(sizes)
         for i, part in enumerate(split(ctx.fetch(n), sizes)):
@@ -148,7 +160,7 @@ def test_tile_batch_equals_single_tiles(ctx):
     assert np.array_equal(pix["iterations_used"], np.concatenate([s.steps for s in singles[1:3]]))
 
 
-def test_all_tiles_of_a_frame(ctx):
+def test_all_tiles_of_a_frame(ctx, batch_kind):
     """The 64 tiles of the 256^2 CPU-runnable configuration, as one batch,
     reassemble the frame rt_trace_camera traces (to the same 1e-12)."""
     base = fixture_task("kerr_centre_tile")

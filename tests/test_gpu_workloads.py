@@ -1,0 +1,201 @@
+"""The tracer's workloads at their full sizes on the GPU, against the oracle.
+
+- Four 2048^2 Kerr frames (a in {0, 0.5, 0.9, 0.998}, inclination 60 deg,
+  r_obs 1000 M, FOV 20 deg) as one rt_trace_batch (per-frame kernels on
+  concurrent streams): every 32nd row of every frame against the oracle at
+  the FP64 parity bar (test_gpu_parity.compare), and property checks on
+  every ray (class set, g per class, step totals equal to the kernel's
+  counters).
+- Mixed precision on Kerr a = 0.998, inclination 85 deg, 4096^2 with the FP64
+  right-hand side inside 3 r_+: classes identical to the FP64 kernel on all
+  16.8 M rays, per-output error against the FP64 kernel within the bars
+  below, and every 64th row against the oracle with the same bars.
+- 512 seeded uneven requests in one batch, on the autotuned shared tile
+  queue and on per-request kernels: every 32nd row of 12 sampled requests
+  against the oracle at the FP64 bar.
+- Serving: Poisson arrivals of the serving mix (30 % superpositions of 2..16
+  holes): every result identical bit for bit to the same request served
+  alone, single-Kerr results within 1e-12 of rt_trace_camera, and every
+  16th row of each superposition against the oracle at the FP64 bar.
+"""
+import math
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import rtsim
+import workloads
+from test_gpu_multi import compare as compare_multi
+from test_gpu_parity import compare, phi_err, rel
+
+pytestmark = pytest.mark.gpu
+
+# Mixed precision (FP32 right-hand side beyond 3 r_+) against FP64 on the
+# precision workload: measured p50 / p99 / max per output on all 16.8 M rays
+# were KS position 8.5e-9 / 7.3e-7 / 3.2e-4, BL theta 5.3e-10 / 4.1e-9 /
+# 9.6e-8, BL phi 2.8e-10 / 2.8e-9 / 3.4e-5, g 9.4e-10 / 9.6e-9 / 1.3e-6
+# (non-captured rays, max(|ref|, 1) denominator). The bars are those with a
+# margin of about 3x; classes must be identical.
+MIXED_BARS = {"x": (3e-8, 3e-6, 1e-3), "theta": (2e-9, 2e-8, 1e-6), "phi": (1e-9, 1e-8, 1e-4),
+              "g": (3e-9, 3e-8, 1e-5)}
+
+
+def rows_of(res, width, rows):
+    """Results of whole rows `rows` of a row-major frame of the given width."""
+    idx = (np.asarray(rows)[:, None] * width + np.arange(width)[None, :]).ravel()
+    sub = rtsim.Results(idx.size)
+    sub.cls[:], sub.lam[:], sub.g[:] = res.cls[idx], res.lam[idx], res.g[idx]
+    sub.steps[:], sub.rejected[:] = res.steps[idx], res.rejected[idx]
+    sub.x[:], sub.p[:], sub.bl[:] = res.x[:, idx], res.p[:, idx], res.bl[:, idx]
+    return sub
+
+
+def oracle_rows(cam, met, ig, tm, stride):
+    count = (cam.height + stride - 1) // stride
+    return O.trace_camera_rows(cam, met, ig, tm, 0, stride, count), list(range(0, cam.height, stride))
+
+
+def properties(res, label):
+    """Size-independent checks on every ray of a frame."""
+    k = res.klass
+    assert set(np.unique(k).tolist()) <= {1, 2, 4}, f"{label}: unclassified rays"
+    assert not (res.cls & rtsim.RT_FLAG_NONFINITE).any(), f"{label}: non-finite rays"
+    assert np.all(res.g[k == 1] == 0.0), f"{label}: captured g"
+    esc = k == 2
+    assert np.allclose(res.g[esc], 1.0 / -res.p[0, esc], rtol=0, atol=1e-15), f"{label}: escaped g"
+    assert np.all((res.g[k == 4] >= 0.05) & (res.g[k == 4] <= 4.0)), f"{label}: disk g outside the clamp"
+    assert np.all(np.isfinite(res.x[:, ~(k == 1)])), f"{label}: non-finite end points"
+
+
+class queue_shape:
+    """Set the context's batch launch shape for a block, restore it after."""
+
+    def __init__(self, ctx, shape):
+        self.ctx, self.shape = ctx, shape
+
+    def __enter__(self):
+        _, self.keep = self.ctx.launch_shapes()
+        self.ctx.set_launch_shapes(queue=self.shape)
+
+    def __exit__(self, *exc):
+        self.ctx.set_launch_shapes(queue=self.keep)
+
+
+def test_frames4_batch_vs_oracle(ctx):
+    reqs = [rtsim.make_request(*r) for r in workloads.frame_requests(2048)]
+    with queue_shape(ctx, rtsim.rt_launch_shape(64, 1, 16, 2)):
+        outs = ctx.trace_batch(reqs)
+    st = ctx.stats()
+    assert st.rays == 4 * 2048 * 2048
+    assert st.accepted_steps == sum(int(o.steps.sum()) for o in outs)
+    assert st.rejected_steps == sum(int(o.rejected.sum()) for o in outs)
+    for r, o in zip(reqs, outs):
+        a = r.metric.spin
+        properties(o, f"frame a={a}")
+        ref, rows = oracle_rows(r.camera, r.metric, r.integrator, r.termination, 32)
+        compare(rows_of(o, 2048, rows), ref, f"2048^2 a={a} every 32nd row")
+        assert {1, 2, 4} <= set(ref.klass.tolist())
+
+
+def mixed_errors(mx, ref):
+    ok = ref.klass != rtsim.RT_CLS_CAPTURED
+    return {"x": (rel(mx.x[1:, ok], ref.x[1:, ok])).max(axis=0),
+            "theta": rel(mx.bl[1, ok], ref.bl[1, ok]),
+            "phi": phi_err(mx.bl[2, ok], ref.bl[2, ok]),
+            "g": rel(mx.g[ok], ref.g[ok])}
+
+
+def check_mixed_bars(errs, label):
+    for k, v in errs.items():
+        p50, p99, mx = np.median(v), np.percentile(v, 99), v.max()
+        b50, b99, bmax = MIXED_BARS[k]
+        print(f"[{label}] {k}: p50 {p50:.2e} p99 {p99:.2e} max {mx:.2e} share<=1e-9 {(v <= 1e-9).mean():.3f}",
+              flush=True)
+        assert p50 < b50 and p99 < b99 and mx < bmax, f"{label}: {k} outside the mixed-precision bars"
+
+
+def test_precision_workload_mixed_vs_fp64(ctx):
+    cam, met, ig64, tm = workloads.precision_request(4096)
+    igmx = workloads.precision_request(4096, rtsim.RT_PRECISION_MIXED)[2]
+    fp64 = ctx.trace_camera(cam, met, ig64, tm)
+    mixed = ctx.trace_camera(cam, met, igmx, tm)
+    st = ctx.stats()
+    assert st.fp32_steps > 0.5 * (st.accepted_steps + st.rejected_steps)
+    assert np.array_equal(mixed.klass, fp64.klass), "mixed precision changed a class"
+    properties(mixed, "mixed 4096^2")
+    check_mixed_bars(mixed_errors(mixed, fp64), "mixed vs FP64 kernel, all rays")
+    ref, rows = oracle_rows(cam, met, ig64, tm, 64)
+    sub = rows_of(mixed, 4096, rows)
+    assert np.array_equal(sub.klass, ref.klass)
+    check_mixed_bars(mixed_errors(sub, ref), "mixed vs oracle, every 64th row")
+    compare(rows_of(fp64, 4096, rows), ref, "FP64 4096^2 every 64th row")
+
+
+SAMPLE = [0, 1, 37, 64, 101, 200, 255, 301, 377, 420, 480, 511]
+
+
+@pytest.mark.parametrize("kind", ["autotuned_tile_queue", "per_request_kernels"])
+def test_uneven512_vs_oracle(ctx, kind):
+    reqs = workloads.uneven_requests()
+    if kind == "autotuned_tile_queue":
+        _, keep = ctx.launch_shapes()
+        try:
+            best = ctx.autotune(reqs[::16])
+            print(f"autotuned block={best.block} waves={best.waves} queue_kind={best.queue_kind}", flush=True)
+            with queue_shape(ctx, rtsim.rt_launch_shape(best.block, best.waves, best.refill_min, 0)):
+                outs = ctx.trace_batch(reqs, only=SAMPLE)
+        finally:
+            ctx.set_launch_shapes(queue=keep)
+    else:
+        with queue_shape(ctx, rtsim.rt_launch_shape(64, 1, 16, 2)):
+            outs = ctx.trace_batch(reqs, only=SAMPLE)
+    st = ctx.stats()
+    assert st.rays == sum(rtsim.n_rays_of(r.camera) for r in reqs)
+    for i in SAMPLE:
+        r, o = reqs[i], outs[i]
+        properties(o, f"request {i}")
+        ref, rows = oracle_rows(r.camera, r.metric, r.integrator, r.termination, 32)
+        compare(rows_of(o, r.camera.width, rows), ref, f"{kind} request {i} ({r.camera.width}^2, "
+                                                       f"rtol {r.integrator.rtol:g})")
+
+
+def serve(reqs, arrivals):
+    """Submit at the given arrival times; returns results in request order."""
+    got = [None] * len(reqs)
+    with rtsim.Server(0, hbm_budget_bytes=1 << 30, max_inflight=128) as srv:
+        ids = {}
+        t0 = time.perf_counter()
+        i = 0
+        while i < len(reqs) or ids:
+            now = time.perf_counter() - t0
+            while i < len(reqs) and arrivals[i] <= now:
+                ids[srv.submit(reqs[i])] = i
+                i += 1
+            for rid in [r for r in ids if srv.poll(r)]:
+                got[ids.pop(rid)] = srv.take(rid)
+            time.sleep(0.0005)
+    return got
+
+
+def test_serving_mix_alone_and_oracle(ctx):
+    reqs = workloads.serving_requests(40, max_rays=512 * 512)
+    arrivals = workloads.poisson_arrivals(len(reqs), 50.0)
+    multi = [i for i, r in enumerate(reqs) if r.metric.n_sources >= 2]
+    assert 5 <= len(multi) <= 25
+    mix = serve(reqs, arrivals)
+    for i, r in enumerate(reqs):
+        alone = serve([r], [0.0])[0]
+        label = f"request {i} ({rtsim.n_rays_of(r.camera)} rays, {max(1, r.metric.n_sources)} sources)"
+        for f in ["cls", "steps", "rejected", "x", "p", "bl", "lam", "g"]:
+            assert np.array_equal(getattr(mix[i], f), getattr(alone, f)), f"{label}: {f} differs alone vs mix"
+        if r.metric.n_sources >= 2:
+            ref, rows = oracle_rows(r.camera, r.metric, r.integrator, r.termination, 16)
+            compare_multi(rows_of(mix[i], r.camera.width, rows), ref, label)
+        else:
+            single = ctx.trace_camera(r.camera, r.metric, r.integrator, r.termination)
+            assert np.array_equal(mix[i].cls, single.cls), label
+            for f in ["x", "p", "bl", "lam", "g"]:
+                d = rel(getattr(mix[i], f), getattr(single, f)).max(initial=0)
+                assert d < 1e-12, f"{label}: {f} differs from rt_trace_camera by {d:.2e}"
