@@ -16,8 +16,10 @@ attempted steps x FLOP_ATTEMPT, op-counted on the CPU oracle) over the
 launch's HIP-event duration on its stream, against the MI355X FP64 vector
 peak. A batch is one launch: one replay of its captured graph, whose per-frame
 kernels run as concurrent branches.
-cpu_baseline: the CPU oracle (a clean-room C++ port, FP64) on every k-th row
-of the same frames, all host threads.
+cpu_baseline: the CPU oracle (a clean-room C++ port, FP64), built on the host
+that runs the bench with -O3 -march=native, on every k-th row of the same
+frames, on every CPU this process may run on (affinity mask and cgroup
+quota), with nproc and the CPU model stated.
 """
 from __future__ import annotations
 
@@ -62,14 +64,57 @@ def frame_requests(side: int, precision: int):
     return workloads.frame_requests(side, precision, INCL_DEG)
 
 
+def host_cpus():
+    """The CPUs this process may run on: the affinity mask, bounded by a
+    cgroup CPU quota when one is set; with nproc and the CPU model."""
+    info = {"nproc": os.cpu_count() or 1}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = info["nproc"]
+    usable = info["affinity"]
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_quota_cpus"] = int(quota) / int(period)
+            usable = min(usable, max(1, math.ceil(info["cgroup_quota_cpus"])))
+    except (OSError, ValueError):
+        pass
+    info["threads"] = usable
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info["model"] = model
+    return info
+
+
+def native_oracle():
+    """Build the oracle for this host (-O3 -march=native, as the reference's
+    own build flags) and return its path; None if the build fails."""
+    import subprocess
+
+    r = subprocess.run(["make", "-B", "-s", "-C", os.path.join(REPO, "oracle"), "liboracle_native.so"],
+                       capture_output=True, text=True)
+    path = os.path.join(REPO, "oracle", "liboracle_native.so")
+    return path if r.returncode == 0 and os.path.exists(path) else None
+
+
 def cpu_baseline(side: int, stride: int, threads: int):
     """Time the CPU oracle on rows 0, stride, 2*stride, ... of each frame."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib
 
-    native = os.path.join(REPO, "oracle", "liboracle_native.so")
-    if os.path.exists(native):
+    cpus = host_cpus()
+    threads = threads or cpus["threads"]
+    native = native_oracle()
+    if native:
         oracle_lib.ORACLE_PATH = native
+    flags = "-O3 -march=native" if native else "-O2 (portable; the native build failed)"
     rows = (side + stride - 1) // stride
     n = 0
     t0 = time.perf_counter()
@@ -79,7 +124,9 @@ def cpu_baseline(side: int, stride: int, threads: int):
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"every {stride}th row of the four {side}x{side} frames ({n} rays, {dt:.1f} s), "
-                      f"oracle/{os.path.basename(oracle_lib.ORACLE_PATH)}"}
+                      f"oracle/{os.path.basename(oracle_lib.ORACLE_PATH)} built {flags} "
+                      f"-frounding-math -fno-math-errno -ffp-contract=off on this host",
+            "host": cpus}
 
 
 def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
@@ -345,9 +392,8 @@ def main() -> int:
         print("bench: 32x32 tile tasks", file=sys.stderr, flush=True)
         out["tiles"] = tile_report(args.side)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        print(f"bench: cpu baseline on {threads} threads", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.side, args.cpu_stride, threads)
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.side, args.cpu_stride, args.cpu_threads)
     print(json.dumps(out), flush=True)
     return 0
 

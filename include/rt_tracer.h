@@ -297,16 +297,20 @@ typedef struct rt_tile_task {
  * they return false (short buffer, zero or oversized screen, zero step
  * budget, non-finite observer / tetrad / field of view / mass / spin /
  * escape radius / timestamp, unknown compression mode, header that does not
- * match the tile's geometry), and also for a metric_type outside the Kerr
- * family (1, 2, 3), which this tracer does not integrate. The push constants
- * map as dispatch_fp64_scalar reads them (render/software_compute_engine.hpp:3794-3813):
- * observer_position and the spatial parts of tetrad_e1/e2/e3 -> camera;
+ * match the tile's geometry), and also for values the reference accepts but
+ * this tracer does not define: a metric_type outside the Kerr family
+ * (1, 2, 3), a field of view outside (0, pi), an escape radius that is not
+ * positive. Every task it accepts can be traced. The push constants map as
+ * dispatch_fp64_scalar reads them (render/software_compute_engine.hpp:3794-3813):
+ * observer_position (radius raised to 1.02 r_+, as the reference clamps it)
+ * and the spatial parts of tetrad_e1/e2/e3 -> camera;
  * metric_mass and metric_spin (0 for metric_type 1, clamped to +-0.999 M)
  * -> metric; escape_radius -> termination.escape_radius;
  * max_integration_steps -> integrator.max_steps (other integrator fields:
- * rt_default_integrator); disk on when primary_disk.enabled > 0.5, inner
- * radius ISCO x max(inner_radius_scale, 1), outer radius
- * max(outer_radius_mass_units x M, 1.05 x inner). */
+ * rt_default_integrator); disk on when primary_disk.enabled > 0.5 and
+ * neither radius field is NaN (a NaN radius fails the reference's range test
+ * for every crossing), inner radius ISCO x max(inner_radius_scale, 1), outer
+ * radius max(outer_radius_mass_units x M, 1.05 x inner). */
 rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* out);
 
 /* Trace the rectangle *region of the camera's frame (rows and columns of the

@@ -119,6 +119,11 @@ rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* o
         return fail(RT_ERR_INVALID_ARG, "tile task: metric_type " + std::to_string(t.metric_type) +
                                             " is not Schwarzschild or Kerr (1, 2, 3)");
     t.body_count = load<uint32_t>(push, kBodyCount);
+    // Values the reference accepts but this tracer's contract does not
+    // define: refused here, so that every decoded task can be traced.
+    if (!(fov > 0.0 && fov < 3.14159))
+        return fail(RT_ERR_INVALID_ARG, "tile task: field of view outside (0, pi), which the pinhole camera needs");
+    if (!(escape > 0.0)) return fail(RT_ERR_INVALID_ARG, "tile task: escape radius not positive");
 
     // push constants -> request, as dispatch_fp64_scalar reads them
     // (render/software_compute_engine.hpp:3794-3813)
@@ -136,6 +141,8 @@ rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* o
     c.row_count = 0;
     const double m = std::max(mass, 1e-4);
     const double a = (t.metric_type == 1) ? 0.0 : std::clamp(spin, -0.999 * m, 0.999 * m);
+    // observer radius clamped to 1.02 r_+ (dispatch_fp64_scalar, software_compute_engine.hpp:3916)
+    c.observer[1] = std::max(c.observer[1], 1.02 * (m + std::sqrt(std::max(m * m - a * a, 0.0))));
     rt_kerr_metric(&t.metric, m, a);
     rt_default_integrator(&t.integrator);
     t.integrator.max_steps = max_steps;
@@ -146,7 +153,10 @@ rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* o
     const double inner_scale = std::max(load<float>(disk, kDiskInnerScale), 1.0f);
     const double outer_units = load<float>(disk, kDiskOuterUnits);
     const double inner = kerr_isco_host(m, a) * inner_scale;
-    t.termination.disk_enabled = (enabled > 0.5f) ? 1u : 0u;
+    // a NaN radius makes the reference's range test fail for every crossing:
+    // the disk is never hit, which is the disk switched off
+    const bool radii_defined = !std::isnan(inner_scale) && !std::isnan(outer_units);
+    t.termination.disk_enabled = (enabled > 0.5f && radii_defined) ? 1u : 0u;
     t.termination.disk_inner = (inner_scale > 1.0) ? inner : 0.0;  // 0: the ISCO itself
     t.termination.disk_outer = std::max(outer_units * m, inner * 1.05);
     *out = t;

@@ -8,6 +8,7 @@ import ctypes as C
 import json
 import math
 import os
+import struct
 import zlib
 
 import numpy as np
@@ -112,7 +113,8 @@ def test_random_mutations_match_reference_verdict():
     """Random byte changes over the header, the timestamp and the first 800
     bytes of the push constants: the decoder accepts exactly what the
     reference's deserialize_render_tile_task accepts, except a metric_type
-    outside 1..3."""
+    outside 1..3, a field of view outside (0, pi) and an escape radius that
+    is not positive."""
     ref = O.ref()
     ref.ref_accepts_tile_task.argtypes = [C.c_void_p, C.c_size_t]
     ref.ref_accepts_tile_task.restype = C.c_int
@@ -129,7 +131,9 @@ def test_random_mutations_match_reference_verdict():
         data = bytes(data)
         want = bool(ref.ref_accepts_tile_task(data, len(data)))
         metric_type = int.from_bytes(data[mt_off:mt_off + 4], "little")
-        want = want and 1 <= metric_type <= 3
+        fov = struct.unpack_from("<d", data, push + FIXTURE["layout"]["field_of_view_rad"])[0]
+        escape = struct.unpack_from("<d", data, push + FIXTURE["layout"]["escape_radius"])[0]
+        want = want and 1 <= metric_type <= 3 and 0.0 < fov < 3.14159 and escape > 0.0
         try:
             rtsim.decode_tile_task(data)
             got = True
@@ -138,3 +142,37 @@ def test_random_mutations_match_reference_verdict():
         n_refused += not got
         assert got == want, data[:32].hex()
     assert n_refused > 100
+
+
+def changed(name, field, fmt, value, extra=0):
+    """Wire bytes of fixture task `name` with one push-constant field (at
+    byte `extra` inside it) replaced."""
+    data = bytearray(task_bytes(next(t for t in tasks(True) if t["name"] == name)))
+    struct.pack_into(fmt, data, FIXTURE["layout"]["push_offset"] + FIXTURE["layout"][field] + extra, value)
+    return bytes(data)
+
+
+def test_changed_fields_decode_to_traceable_tasks():
+    """Single fields of an accepted task changed to values the reference
+    accepts: each decodes to a task the tracer accepts (the reference's clamp
+    of the observer radius, the disk off where a NaN radius means it is never
+    hit), or is refused with a message (field of view outside (0, pi),
+    escape radius not positive)."""
+    base = rtsim.decode_tile_task(changed("kerr_centre_tile", "escape_radius", "<d", 1000.0))
+    m, a = base.metric.mass, base.metric.spin
+    r_h = m + math.sqrt(m * m - a * a)
+    for r in (0.0, -5.0, 0.5 * r_h):  # observer_position[1] is r
+        t = rtsim.decode_tile_task(changed("kerr_centre_tile", "observer_position", "<d", r, extra=8))
+        assert t.camera.observer[1] == pytest.approx(1.02 * r_h, rel=1e-15)
+    for fov in (0.0, -0.1, math.pi, 4.0):
+        with pytest.raises(rtsim.RtError, match="field of view"):
+            rtsim.decode_tile_task(changed("kerr_centre_tile", "field_of_view_rad", "<d", fov))
+    for esc in (0.0, -10.0):
+        with pytest.raises(rtsim.RtError, match="escape radius"):
+            rtsim.decode_tile_task(changed("kerr_centre_tile", "escape_radius", "<d", esc))
+    for field in ("disk_outer_radius_mass_units", "disk_inner_radius_scale"):
+        t = rtsim.decode_tile_task(changed("kerr_centre_tile", field, "<f", float("nan")))
+        assert t.termination.disk_enabled == 0
+        t = rtsim.decode_tile_task(changed("kerr_centre_tile", field, "<f", 2.0))
+        assert t.termination.disk_enabled == base.termination.disk_enabled
+        assert math.isfinite(t.termination.disk_inner) and math.isfinite(t.termination.disk_outer)
