@@ -26,6 +26,9 @@
  *                        (include/relativistic/render/software_compute_engine.hpp:905-1022)
  *   rt_trace_tiles       many tile tasks in one dispatch: the task loop of
  *                        TileRenderer::render (src/distributed/tile_renderer.cpp:133-182)
+ *   RT_METHOD_RENDER_RK4 the renderer's own tracer, trace_kerr_schild_photon
+ *                        (include/relativistic/render/software_compute_engine.hpp:1624-1809)
+ *                        as dispatch_fp64 runs it for a Kerr frame
  *   rt_server_*          the windowed worker of GeodesicComputePipeline
  *                        (include/relativistic/render/geodesic_compute_pipeline.hpp:529-728):
  *                        requests, cancellation, completion polling
@@ -40,7 +43,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 1u
+#define RT_ABI_VERSION 2u
 #define RT_MAX_SOURCES 16u
 
 /* ------------------------------------------------------------------ status */
@@ -102,6 +105,33 @@ typedef struct rt_metric {
     rt_source src[RT_MAX_SOURCES];
 } rt_metric;
 
+/* Integration method. */
+typedef enum rt_method {
+    RT_METHOD_DP45 = 0,       /* adaptive Dormand–Prince 5(4) (the default)       */
+    /* The reference renderer's tracer (trace_kerr_schild_photon,
+     * render/software_compute_engine.hpp:1624-1809, as dispatch_fp64 runs it
+     * for a Kerr frame): classic RK4 with the render step
+     *   |h| = clamp(step_size_factor sqrt(r max(r - r_+, 0.02 M)) F(r),
+     *               min_step, max_step far_field_step_scale),
+     *   F(r) = 1 + (far_field_step_scale - 1) clamp((r - 20 r_+) / (80 r_+), 0, 1)
+     * on BL r; capture at r <= 1.0001 r_+ (or a non-finite state), escape at
+     * r >= escape_radius; a translucent disk: every crossing of theta = pi/2
+     * with r in [disk_inner, disk_outer] and r > 1.02 r_+ sets the disk bit,
+     * records g and multiplies the throughput by 1 - opacity (the disk
+     * profile's), and the ray ends once the throughput is <= 0.01; after the
+     * loop, a ray neither captured nor escaped inside 3 r_s has throughput 0,
+     * and the escaped bit is set when escaped or throughput > 0.01.
+     * max_steps is the reference's max_integration_steps, halved with a floor
+     * of 256 as dispatch_fp64_scalar does for the exact Kerr path. Results:
+     * cls = the reference's status_flags (any combination of 1, 2, 4), steps
+     * = iterations_used (loop entries), lambda = iterations_used x 0.05 (its
+     * affine_parameter), g = its redshift (1 unless a disk crossing set it),
+     * x and p the end state (p_t = -1), bl the end point with the azimuth
+     * continued from the observer's. FP64 only; single Kerr only; cameras
+     * only (not explicit rays). */
+    RT_METHOD_RENDER_RK4 = 1
+} rt_method;
+
 typedef struct rt_integrator {
     double rtol;          /* DP45 relative tolerance (1e-10)                      */
     double atol;          /* DP45 absolute tolerance (1e-12)                      */
@@ -113,6 +143,14 @@ typedef struct rt_integrator {
                                    raises it to at least 3 r_+                    */
     uint32_t max_steps;   /* cap on attempted (accepted + rejected) steps         */
     uint32_t precision;   /* rt_precision                                         */
+    uint32_t method;      /* rt_method                                            */
+    uint32_t reserved;
+    /* RT_METHOD_RENDER_RK4 step formula (GpuCameraPushConstants fields of the
+     * same names, render/gpu_types.hpp:966-1109; defaults 0.5, 0.004, 3.5, 7.8) */
+    double step_size_factor;
+    double min_step;
+    double max_step;
+    double far_field_step_scale;
 } rt_integrator;
 
 #define RT_TERM_NO_G_CLAMP 1u /* do not clamp disk g to [0.05, 4]                */
@@ -127,6 +165,25 @@ typedef struct rt_integrator {
  * Off by default. */
 #define RT_TERM_CARTER_CAPTURE 2u
 
+/* The disk's appearance parameters: the float fields of the reference's
+ * GpuDiskProfile (render/gpu_types.hpp:42-77) in its order, 128 bytes, so a
+ * caller can copy that struct in. RT_METHOD_RENDER_RK4 reads the ones that
+ * decide the disk's opacity (AccretionDiskModel::shade,
+ * render/accretion_disk/accretion_disk_model.hpp:245-314): opacity_scale,
+ * edge_softness, rotation_speed_scale, the ring, spiral, turbulence and
+ * grain fields and noise_seed. The disk's on/off switch and radii are
+ * rt_termination's disk_enabled, disk_inner and disk_outer. */
+typedef struct rt_disk_profile {
+    float enabled, inner_radius_scale, outer_radius_mass_units, peak_temperature_k;
+    float floor_temperature_k, temperature_exponent, zero_torque_strength, extra_beaming_exponent;
+    float brightness, color_saturation, tint_strength, opacity_scale;
+    float tint_r, tint_g, tint_b, rotation_speed_scale;
+    float ring_amplitude, ring_frequency, ring_sharpness, spiral_arm_count;
+    float spiral_pitch, spiral_amplitude, turbulence_amplitude, turbulence_scale;
+    float turbulence_octaves, radial_stretch, grain_amplitude, grain_scale;
+    float reference_luminance, noise_seed, temperature_normalization, edge_softness;
+} rt_disk_profile;
+
 typedef struct rt_termination {
     double horizon_eps;   /* captured when r <= r_+ + horizon_eps (1e-6)           */
     double escape_radius; /* escaped when r >= escape_radius while outgoing (1000) */
@@ -134,21 +191,41 @@ typedef struct rt_termination {
     double disk_outer;    /* equatorial disk outer radius (20 M)                  */
     uint32_t disk_enabled;
     uint32_t flags;       /* RT_TERM_* bits                                        */
+    double time;          /* RT_METHOD_RENDER_RK4: the disk pattern's time (push
+                             constant `time`; rotates the opacity pattern)       */
+    rt_disk_profile disk_profile; /* RT_METHOD_RENDER_RK4: opacity parameters     */
 } rt_termination;
 
+/* Camera projections: the reference's ProjectionMode values
+ * (observer/camera_projections.hpp:13-22, CameraProjector::compute_ray_direction
+ * :198-225, as dispatch_fp64_scalar calls it with no boost). The all-sky
+ * modes (3, 7) take the horizontal screen coordinate without the aspect
+ * ratio (software_compute_engine.hpp:3855-3856). */
+#define RT_PROJ_PINHOLE 0u
+#define RT_PROJ_AUTO_ZOOM 1u          /* with no observer boost: the pinhole   */
+#define RT_PROJ_FISHEYE_STEREOGRAPHIC 2u
+#define RT_PROJ_EQUIRECTANGULAR_360 3u
+#define RT_PROJ_FISHEYE_EQUIDISTANT 4u
+#define RT_PROJ_FISHEYE_ORTHOGRAPHIC 5u
+#define RT_PROJ_PANINI_CYLINDRICAL 6u
+#define RT_PROJ_HAMMER_AITOFF 7u
+
 typedef struct rt_camera {
-    /* Pinhole camera, as GpuCameraPushConstants (render/gpu_types.hpp:966-1109):
+    /* Camera, as GpuCameraPushConstants (render/gpu_types.hpp:966-1109):
      * observer_position (t, r, theta, phi) and the spatial parts of
      * tetrad_e1 (forward), tetrad_e2 (right), tetrad_e3 (up), world Cartesian. */
     double observer[4];
     double forward[3];
     double right[3];
     double up[3];
-    double fov_y;         /* vertical full field of view, radians                 */
+    double fov_y;         /* field of view, radians (the pinhole's vertical full
+                             angle; each projection reads it as the reference does) */
     uint32_t width;
     uint32_t height;
     uint32_t row_offset;  /* first traced row (dispatch_row_offset)               */
     uint32_t row_count;   /* traced rows; 0 = all (dispatch_row_count)            */
+    uint32_t projection;  /* RT_PROJ_* (projection_mode); 0 = pinhole           */
+    uint32_t reserved;
 } rt_camera;
 
 typedef struct rt_results {

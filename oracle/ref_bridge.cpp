@@ -6,6 +6,8 @@
 // tests/golden/make_golden.py to pin the clean-room oracle to the reference's
 // own functions; nothing here is reference code, only calls into it.
 #include <cmath>
+#include <cstddef>
+#include <cstring>
 #include <cstdint>
 
 #include "relativistic/core/kerr_schild_null_integrator.hpp"
@@ -15,6 +17,7 @@
 #include "relativistic/render/accretion_disk/accretion_disk_model.hpp"
 #include "relativistic/render/software_compute_engine.hpp"
 #include "relativistic/distributed/task_payloads.hpp"
+#include "relativistic/observer/camera_projections.hpp"
 
 #include <span>
 #include <vector>
@@ -323,6 +326,85 @@ size_t ref_make_tile_task(const double* d, const uint32_t* u, const float* f, ui
     if (!h) return 0;
     task.header = *h;
     return Distributed::serialize_render_tile_task(task, std::span<uint8_t>(out, cap));
+}
+
+// CameraProjector<double>::compute_ray_direction (observer/camera_projections.hpp:198-225)
+// with the arguments dispatch_fp64_scalar passes (no boost).
+void ref_screen_direction(uint32_t mode, double u, double v, double fov, double out[3]) {
+    const auto d = Observer::CameraProjector<double>::compute_ray_direction(static_cast<Observer::ProjectionMode>(mode),
+                                                                           u, v, fov);
+    out[0] = d[0];
+    out[1] = d[1];
+    out[2] = d[2];
+}
+
+// SoftwareComputeEngine::dispatch_fp64 (render/software_compute_engine.hpp:5546-5587)
+// on a Kerr frame given in the C ABI's terms: the push constants take the
+// camera (observer, tetrad columns, field of view, size, projection, row
+// window), the metric (metric_type 2, mass, spin), escape radius, step
+// formula, max_integration_steps, time and the disk profile (enabled from
+// disk_enabled; inner_radius_scale = disk_inner / ISCO, 1 when disk_inner is
+// 0; outer_radius_mass_units = disk_outer / M, as floats). Everything else
+// keeps its default. Writes the GpuPixelOutput fields of each traced pixel.
+void ref_render_frame(const rt_camera* cam, const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm,
+                      uint32_t* status, uint32_t* iters, float* redshift, float* affine, float* rgb) {
+    Render::GpuCameraPushConstants pc{};
+    for (int i = 0; i < 4; ++i) pc.observer_position[i] = cam->observer[i];
+    pc.tetrad_e1 = {0.0, cam->forward[0], cam->forward[1], cam->forward[2]};
+    pc.tetrad_e2 = {0.0, cam->right[0], cam->right[1], cam->right[2]};
+    pc.tetrad_e3 = {0.0, cam->up[0], cam->up[1], cam->up[2]};
+    pc.field_of_view_rad = cam->fov_y;
+    pc.screen_width = cam->width;
+    pc.screen_height = cam->height;
+    pc.projection_mode = cam->projection;
+    pc.dispatch_row_offset = cam->row_offset;
+    pc.dispatch_row_count = cam->row_count;
+    pc.metric_type = 2U;
+    pc.metric_mass = mt->mass;
+    pc.metric_spin = mt->spin;
+    pc.escape_radius = tm->escape_radius;
+    pc.max_integration_steps = ig->max_steps;
+    pc.step_size_factor = ig->step_size_factor;
+    pc.min_step_size = ig->min_step;
+    pc.max_step_size = ig->max_step;
+    pc.far_field_step_scale = ig->far_field_step_scale;
+    pc.time = tm->time;
+    static_assert(sizeof(Render::GpuDiskProfile) == sizeof(rt_disk_profile), "disk profile layout");
+    std::memcpy(&pc.primary_disk, &tm->disk_profile, sizeof(rt_disk_profile));
+    const double m = std::max(mt->mass, 1e-4);
+    const double a = std::clamp(mt->spin, -0.999 * m, 0.999 * m);
+    const double isco = Optics::DiskThermalProfile::kerr_isco_radius(m, a);
+    pc.primary_disk.enabled = tm->disk_enabled ? 1.0f : 0.0f;
+    pc.primary_disk.inner_radius_scale = tm->disk_inner > 0.0 ? static_cast<float>(tm->disk_inner / isco) : 1.0f;
+    pc.primary_disk.outer_radius_mass_units = static_cast<float>((tm->disk_outer > 0.0 ? tm->disk_outer : 20.0 * m) / m);
+    const uint32_t first = std::min(cam->row_offset, cam->height);
+    const uint32_t last = cam->row_count ? std::min(first + cam->row_count, cam->height) : cam->height;
+    const size_t n = static_cast<size_t>(last - first) * cam->width;
+    std::vector<Render::GpuPixelOutput> px(n);
+    Render::SoftwareComputeEngine::dispatch_fp64(pc, std::span<Render::GpuPixelOutput>(px.data(), n));
+    for (size_t i = 0; i < n; ++i) {
+        status[i] = px[i].status_flags;
+        iters[i] = px[i].iterations_used;
+        redshift[i] = px[i].redshift;
+        affine[i] = px[i].affine_parameter;
+        if (rgb) {
+            rgb[3 * i] = px[i].r;
+            rgb[3 * i + 1] = px[i].g;
+            rgb[3 * i + 2] = px[i].b;
+        }
+    }
+}
+
+// offsetof(GpuPixelOutput, field) for r, g, b, a, redshift, affine_parameter,
+// status_flags, iterations_used, then sizeof (render/gpu_types.hpp:1111-1121).
+int ref_pixel_layout(uint64_t* out) {
+    using Render::GpuPixelOutput;
+    const uint64_t v[] = {offsetof(GpuPixelOutput, r), offsetof(GpuPixelOutput, g), offsetof(GpuPixelOutput, b),
+                          offsetof(GpuPixelOutput, a), offsetof(GpuPixelOutput, redshift),
+                          offsetof(GpuPixelOutput, affine_parameter), offsetof(GpuPixelOutput, status_flags),
+                          offsetof(GpuPixelOutput, iterations_used), sizeof(GpuPixelOutput)};
+    for (int i = 0; i < 9; ++i) out[i] = v[i];
+    return 9;
 }
 
 // deserialize_render_tile_task (task_payloads.cpp:114-129): 1 when it

@@ -614,32 +614,135 @@ static void ms_initial_momentum(const Spacetime& st, const double x[3], const do
 }
 
 // -------------------------------------------------------------- camera
-// Pixel -> initial state, restating dispatch_fp64_scalar (software_compute_engine.hpp:3843-3938),
-// CameraProjector::compute_pinhole_ray (observer/camera_projections.hpp:27-41) and the
-// set-up of trace_kerr_schild_photon (:1634-1652) / trace_multi_source (:3054-3066).
-// Returns false when the observer sits inside a horizon (captured at once).
-bool camera_ray(const Spacetime& st, const rt_camera* cam, uint32_t px, uint32_t py, double y[8], double* xi) {
+// Unit vector (n1, n2, n3) = (forward, up, right) components of the ray
+// through screen point (u, v), restating CameraProjector::compute_ray_direction
+// (observer/camera_projections.hpp:198-225) and its projections (:27-196)
+// for the call dispatch_fp64_scalar makes (no boost: the auto-zoom ray is the
+// pinhole one). Unknown modes fall back to the pinhole, as the switch does.
+static void normalized(double n1, double n2, double n3, double nl[3]) {
+    const double len = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
+    const double inv_len = (len > 0.0) ? (1.0 / len) : 1.0;
+    nl[0] = n1 * inv_len;
+    nl[1] = n2 * inv_len;
+    nl[2] = n3 * inv_len;
+}
+
+void screen_direction(uint32_t mode, double u, double v, double fov, double nl[3]) {
+    switch (mode) {
+        case RT_PROJ_FISHEYE_STEREOGRAPHIC: {  // :70-94
+            const double max_fov = std::clamp(fov, 0.1, 260.0 * kPi / 180.0);
+            const double r = sqrt(u * u + v * v);
+            if (r < 1e-15) {
+                nl[0] = 1.0, nl[1] = 0.0, nl[2] = 0.0;
+                return;
+            }
+            const double half_max_fov = max_fov * 0.5;
+            const double theta = 2.0 * std::atan(r * std::tan(half_max_fov * 0.5));
+            const double sin_t = std::sin(theta), cos_t = std::cos(theta);
+            normalized(cos_t, v * (sin_t / r), u * (sin_t / r), nl);
+            return;
+        }
+        case RT_PROJ_EQUIRECTANGULAR_360: {  // :174-196
+            const double fov_scale = fov / (60.0 * kPi / 180.0);
+            const double phi = u * kPi * fov_scale;
+            const double theta = std::clamp(kPi * 0.5 - v * (kPi * 0.5) * fov_scale, 0.0001, kPi - 0.0001);
+            const double sin_t = std::sin(theta), cos_t = std::cos(theta);
+            const double sin_p = std::sin(phi), cos_p = std::cos(phi);
+            normalized(sin_t * cos_p, cos_t, sin_t * sin_p, nl);
+            return;
+        }
+        case RT_PROJ_FISHEYE_EQUIDISTANT: {  // :96-113
+            const double r = sqrt(u * u + v * v);
+            if (r < 1e-15) {
+                nl[0] = 1.0, nl[1] = 0.0, nl[2] = 0.0;
+                return;
+            }
+            const double theta = min(r * fov * 0.5, kPi);
+            const double sin_t = std::sin(theta), cos_t = std::cos(theta);
+            normalized(cos_t, v * (sin_t / r), u * (sin_t / r), nl);
+            return;
+        }
+        case RT_PROJ_FISHEYE_ORTHOGRAPHIC: {  // :115-132
+            const double r = sqrt(u * u + v * v);
+            const double sin_half_fov = std::sin(std::clamp(fov * 0.5, 0.01, kPi * 0.499));
+            const double sin_t = std::clamp(r * sin_half_fov, 0.0, 1.0);
+            const double cos_t = sqrt(max(1.0 - sin_t * sin_t, 0.0));
+            if (r < 1e-15) {
+                nl[0] = 1.0, nl[1] = 0.0, nl[2] = 0.0;
+                return;
+            }
+            normalized(cos_t, v * (sin_t / r), u * (sin_t / r), nl);
+            return;
+        }
+        case RT_PROJ_PANINI_CYLINDRICAL: {  // :134-152
+            const double d = 1.0;
+            const double tan_h = std::tan(fov * 0.5);
+            const double px = u * tan_h, py = v * tan_h;
+            const double s = (d + 1.0) / (d + sqrt(max(1.0 + (1.0 - d * d) * px * px / ((d + 1.0) * (d + 1.0)), 1e-15)));
+            const double sin_phi = (px * s) / (d + 1.0);
+            const double cos_phi = sqrt(max(1.0 - sin_phi * sin_phi, 0.0));
+            normalized(cos_phi, py, sin_phi, nl);
+            return;
+        }
+        case RT_PROJ_HAMMER_AITOFF: {  // :154-172 (not renormalised)
+            const double sqrt2 = 1.4142135623730951;
+            const double zoom = (60.0 * kPi / 180.0) / max(fov, 0.01);
+            const double x = std::clamp(u * zoom * sqrt2, -sqrt2, sqrt2);
+            const double y = std::clamp(v * zoom * (sqrt2 * 0.5), -(sqrt2 * 0.5), sqrt2 * 0.5);
+            const double z = sqrt(max(1.0 - (x * x * 0.25) - (y * y), 1e-15));
+            const double phi = 2.0 * std::atan2(z * x, 2.0 * (2.0 * z * z - 1.0));
+            const double theta = kPi * 0.5 - std::asin(std::clamp(z * y * sqrt2, -1.0, 1.0));
+            const double sin_t = std::sin(theta), cos_t = std::cos(theta);
+            const double sin_p = std::sin(phi), cos_p = std::cos(phi);
+            nl[0] = sin_t * cos_p;
+            nl[1] = cos_t;
+            nl[2] = sin_t * sin_p;
+            return;
+        }
+        default: {  // pinhole (:27-41), auto-zoom with no boost (:43-68)
+            const double tan_half = std::tan(fov * 0.5);
+            normalized(1.0, v * tan_half, u * tan_half, nl);
+            return;
+        }
+    }
+}
+
+// Pixel -> initial state, restating dispatch_fp64_scalar (software_compute_engine.hpp:3843-3938)
+// with the projections above and the set-up of trace_kerr_schild_photon (:1634-1652) /
+// trace_multi_source (:3054-3066). The all-sky modes take u without the
+// aspect ratio (:3855-3856). Returns false when the observer sits inside a
+// horizon (captured at once).
+void camera_outward(const rt_camera* cam, double rh, uint32_t px, uint32_t py, double outward[3]) {
     const double W = cam->width, Hh = cam->height;
     const double aspect = W / Hh;
     const double v = 1.0 - (static_cast<double>(py) + 0.5) / Hh * 2.0;
-    const double u = ((static_cast<double>(px) + 0.5) / W * 2.0 - 1.0) * aspect;
-    const double tan_half = std::tan(cam->fov_y * 0.5);
-    const double n1 = 1.0, n2 = v * tan_half, n3 = u * tan_half;
-    const double len = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
-    const double inv_len = (len > 0.0) ? (1.0 / len) : 1.0;
-    const double nl[3] = {n1 * inv_len, n2 * inv_len, n3 * inv_len};
+    const double u_norm = ((static_cast<double>(px) + 0.5) / W * 2.0 - 1.0) * aspect;
+    const bool allsky = cam->projection == RT_PROJ_EQUIRECTANGULAR_360 || cam->projection == RT_PROJ_HAMMER_AITOFF;
+    const double u = allsky ? (((static_cast<double>(px) + 0.5) / W) * 2.0 - 1.0) : u_norm;
+    double nl[3];
+    screen_direction(cam->projection, u, v, cam->fov_y, nl);
     const double d[3] = {nl[0] * cam->forward[0] + nl[2] * cam->right[0] + nl[1] * cam->up[0],
                          nl[0] * cam->forward[1] + nl[2] * cam->right[1] + nl[1] * cam->up[1],
                          nl[0] * cam->forward[2] + nl[2] * cam->right[2] + nl[1] * cam->up[2]};
-    const double r_obs = max(cam->observer[1], st.rh * 1.02);
+    (void)rh;
     const double th = std::clamp(cam->observer[2], 0.001, kPi - 0.001);
     const double ph = cam->observer[3];
     const double st_ = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
     const double n_r = d[0] * (st_ * cp) + d[1] * (st_ * sp) + d[2] * ct;
     const double n_th = d[0] * (ct * cp) + d[1] * (ct * sp) + d[2] * (-st_);
     const double n_ph = d[0] * (-sp) + d[1] * cp + d[2] * 0.0;
-    const double outward[3] = {n_r * st_ * cp + n_th * ct * cp - n_ph * sp, n_r * st_ * sp + n_th * ct * sp + n_ph * cp,
-                               n_r * ct - n_th * st_};
+    outward[0] = n_r * st_ * cp + n_th * ct * cp - n_ph * sp;
+    outward[1] = n_r * st_ * sp + n_th * ct * sp + n_ph * cp;
+    outward[2] = n_r * ct - n_th * st_;
+}
+
+bool camera_ray(const Spacetime& st, const rt_camera* cam, uint32_t px, uint32_t py, double y[8], double* xi) {
+    double outward[3];
+    camera_outward(cam, st.rh, px, py, outward);
+    const double r_obs = max(cam->observer[1], st.rh * 1.02);
+    const double th = std::clamp(cam->observer[2], 0.001, kPi - 0.001);
+    const double ph = cam->observer[3];
+    const double st_ = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
     y[0] = 0.0;
     y[4] = -1.0;
     if (!st.multi) {
@@ -700,6 +803,275 @@ static void parallel_rows(size_t rows, int threads, Fn&& fn) {
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
     for (auto& t : pool) t.join();
+}
+
+
+// ------------------------------------------------ the renderer's tracer
+// RT_METHOD_RENDER_RK4: restates trace_kerr_schild_photon
+// (render/software_compute_engine.hpp:1624-1809) without its shading (sky,
+// colour, tonemapping), keeping everything that decides the pixel record's
+// status_flags, iterations_used and redshift: RK4 steps of the render step
+// formula, relative horizon, escape radius, the translucent disk whose
+// opacity (AccretionDiskModel::shade, render/accretion_disk/accretion_disk_model.hpp:245-314)
+// multiplies the throughput, and the after-loop rules. Set-up as
+// dispatch_fp64_scalar (:3794-3832): budget halved with a floor of 256,
+// turbulence anti-aliasing factor of the frame.
+
+struct RenderCfg {
+    double m, a, rs, rh, isco, r_esc, disk_in, disk_out, disk_spin;
+    bool disk_on;
+    uint32_t max_steps;
+    double kappa, h_min, h_max, far_scale, time, detail;
+    rt_disk_profile prof;
+};
+
+static RenderCfg make_render(const rt_camera* cam, const rt_metric* mt, const rt_integrator* ig,
+                             const rt_termination* tm) {
+    RenderCfg c;
+    c.m = max(mt->mass, 1e-4);
+    c.a = std::clamp(mt->spin, -0.999 * c.m, 0.999 * c.m);
+    c.rs = 2.0 * c.m;
+    c.rh = c.m + sqrt(max(c.m * c.m - c.a * c.a, 0.0));
+    c.isco = kerr_isco(c.m, c.a);
+    c.r_esc = tm->escape_radius;
+    c.disk_on = tm->disk_enabled != 0;
+    c.disk_in = (tm->disk_inner > 0.0) ? tm->disk_inner : c.isco;
+    c.disk_out = max((tm->disk_outer > 0.0) ? tm->disk_outer : 20.0 * c.m, c.disk_in * 1.05);
+    c.disk_spin = std::clamp(c.a, -0.999 * c.m, 0.999 * c.m);
+    c.max_steps = std::max(ig->max_steps / 2U, 256U);
+    c.kappa = ig->step_size_factor;
+    c.h_min = ig->min_step;
+    c.h_max = ig->max_step;
+    c.far_scale = ig->far_field_step_scale;
+    c.time = tm->time;
+    c.prof = tm->disk_profile;
+    // turbulence anti-aliasing factor (dispatch_fp64_scalar :3834-3836); its r_h
+    // is 2 M for |a| <= 1e-12, the same number
+    const double rh_frame = (abs(c.a) > 1e-12) ? c.rh : c.rs;
+    const double r_obs_frame = max(cam->observer[1], rh_frame * 1.02);
+    const double angular_pixel_size = cam->fov_y / max(static_cast<double>(cam->width), 1.0);
+    const double bh_angular_diameter = (2.0 * rh_frame) / r_obs_frame;
+    c.detail = std::clamp(bh_angular_diameter / max(angular_pixel_size * 24.0, 1e-9), 0.0, 1.0);
+    return c;
+}
+
+// BL point with the azimuth continued from prev_phi (boyer_lindquist_from_kerr_schild,
+// core/kerr_schild_null_integrator.hpp:56-69).
+static void bl_point(const double x[3], double a, double prev_phi, double out[3]) {
+    const double r = ks_radius(x[0], x[1], x[2], a);
+    const double rho = sqrt(max(r * r - x[2] * x[2], 0.0));
+    const double two_pi = 2.0 * kPi;
+    const double raw_phi = atan2(x[1], x[0]) - atan2(a, r);
+    double delta = raw_phi - prev_phi;
+    delta -= two_pi * floor(delta / two_pi + 0.5);
+    out[0] = r;
+    out[1] = atan2(rho, x[2]);
+    out[2] = prev_phi + delta;
+}
+
+// One classic RK4 step (step_kerr_schild_null_rk4, core/kerr_schild_null_integrator.hpp:116-139)
+// of the KS null state, p_t = -1; y[1..3] position, y[5..7] momentum.
+static void render_rk4(double y[8], double h, double M, double a) {
+    double k1[8], k2[8], k3[8], k4[8], t[8];
+    const double half = 0.5 * h;
+    auto advance = [&](const double* slope, double scale) {
+        t[0] = y[0];
+        t[4] = y[4];
+        for (int i = 1; i <= 3; ++i) {
+            t[i] = y[i] + scale * slope[i];
+            t[4 + i] = y[4 + i] + scale * slope[4 + i];
+        }
+    };
+    ks_rhs(y, M, a, k1);
+    advance(k1, half);
+    ks_rhs(t, M, a, k2);
+    advance(k2, half);
+    ks_rhs(t, M, a, k3);
+    advance(k3, h);
+    ks_rhs(t, M, a, k4);
+    const double sixth = h / 6.0;
+    for (int i = 1; i <= 3; ++i) {
+        y[i] += sixth * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+        y[4 + i] += sixth * (k1[4 + i] + 2.0 * k2[4 + i] + 2.0 * k3[4 + i] + k4[4 + i]);
+    }
+}
+
+// The disk's opacity at a crossing: the opacity path of AccretionDiskModel::shade
+// (render/accretion_disk/accretion_disk_model.hpp:110-176 helpers, :245-314).
+static double smooth_unit(double x) {
+    const double t = std::clamp(x, 0.0, 1.0);
+    return t * t * (3.0 - 2.0 * t);
+}
+
+static double reduce_angle(double angle) {
+    const double two_pi = 2.0 * kPi;
+    return angle - two_pi * floor(angle / two_pi);
+}
+
+static uint32_t hash_u32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+static float lattice_value(int32_t x, int32_t y, int32_t z, uint32_t seed) {
+    const uint32_t h = static_cast<uint32_t>(x) * 374761393U + static_cast<uint32_t>(y) * 668265263U +
+                       static_cast<uint32_t>(z) * 2147483647U + seed * 1274126177U;
+    return static_cast<float>(hash_u32(h) & 0x00FFFFFFU) * (1.0f / 16777216.0f) * 2.0f - 1.0f;
+}
+
+static float value_noise(float x, float y, float z, uint32_t seed) {
+    const float fx0 = std::floor(x), fy0 = std::floor(y), fz0 = std::floor(z);
+    const int32_t xi = static_cast<int32_t>(fx0), yi = static_cast<int32_t>(fy0), zi = static_cast<int32_t>(fz0);
+    const float fx = x - fx0, fy = y - fy0, fz = z - fz0;
+    const float ux = fx * fx * (3.0f - 2.0f * fx);
+    const float uy = fy * fy * (3.0f - 2.0f * fy);
+    const float uz = fz * fz * (3.0f - 2.0f * fz);
+    const float c000 = lattice_value(xi, yi, zi, seed), c100 = lattice_value(xi + 1, yi, zi, seed);
+    const float c010 = lattice_value(xi, yi + 1, zi, seed), c110 = lattice_value(xi + 1, yi + 1, zi, seed);
+    const float c001 = lattice_value(xi, yi, zi + 1, seed), c101 = lattice_value(xi + 1, yi, zi + 1, seed);
+    const float c011 = lattice_value(xi, yi + 1, zi + 1, seed), c111 = lattice_value(xi + 1, yi + 1, zi + 1, seed);
+    const float c00 = c000 + (c100 - c000) * ux, c10 = c010 + (c110 - c010) * ux;
+    const float c01 = c001 + (c101 - c001) * ux, c11 = c011 + (c111 - c011) * ux;
+    const float c0 = c00 + (c10 - c00) * uy, c1 = c01 + (c11 - c01) * uy;
+    return c0 + (c1 - c0) * uz;
+}
+
+static float fractal_noise(float x, float y, float z, int octaves, uint32_t seed) {
+    float total = 0.0f, amplitude = 1.0f, frequency = 1.0f, normalization = 0.0f;
+    for (int o = 0; o < octaves; ++o) {
+        total += value_noise(x * frequency, y * frequency, z * frequency, seed + static_cast<uint32_t>(o) * 7919U) *
+                 amplitude;
+        normalization += amplitude;
+        amplitude *= 0.5f;
+        frequency *= 2.0f;
+    }
+    return total / std::max(normalization, 1e-5f);
+}
+
+// Keplerian angular velocity of AccretionDiskModel::orbital_frame (:205-216).
+static double disk_angular_velocity(double mass, double spin, double radius) {
+    const double spin_mag = abs(spin);
+    const double sqrt_m = sqrt(max(mass, 1e-12));
+    const double sqrt_r = sqrt(max(radius, 1e-12));
+    const double angular = sqrt_m / (radius * sqrt_r + spin_mag * sqrt_m);
+    return (spin < 0.0) ? -angular : angular;
+}
+
+static double disk_opacity(const rt_disk_profile& pf, double m, double spin, double inner_r, double outer_r,
+                           double radius_in, double azimuth, double time, double detail_scale) {
+    const double mass = max(m, 1e-9);
+    const double radius = max(radius_in, 1e-9);
+    const double inner = max(inner_r, 1e-9);
+    const double outer = max(outer_r, inner * 1.0001);
+    const double softness = max(static_cast<double>(pf.edge_softness), 0.05);
+    const double envelope =
+        smooth_unit((radius - inner) / (0.8 * softness * mass)) * smooth_unit((outer - radius) / (1.5 * softness * mass));
+    const double rotation = static_cast<double>(pf.rotation_speed_scale) * time;
+    const double shear_phase = reduce_angle(azimuth - disk_angular_velocity(mass, spin, radius) * rotation);
+    const double pattern_phase = reduce_angle(azimuth - disk_angular_velocity(mass, spin, sqrt(inner * outer)) * rotation);
+    const float log_ratio = static_cast<float>(std::log(max(radius / inner, 1.0)));
+    const float log_span = std::max(static_cast<float>(std::log(max(outer / inner, 1.0001))), 1e-4f);
+    const float radial = log_ratio / log_span;
+    const float pi_f = 3.14159265358979323846f;
+    const float two_pi = 2.0f * pi_f;
+    const float ring_wave = 0.5f + 0.5f * std::cos(two_pi * pf.ring_frequency * radial);
+    const float ring_sharpness = std::max(pf.ring_sharpness, 1.0f);
+    const float ring = std::max(
+        1.0f + pf.ring_amplitude * (std::pow(ring_wave, ring_sharpness) * std::sqrt(pi_f * ring_sharpness) - 1.0f), 0.0f);
+    const float arms = std::floor(pf.spiral_arm_count + 0.5f);
+    const float spiral =
+        (arms > 0.5f)
+            ? std::max(1.0f + pf.spiral_amplitude * std::cos(arms * (static_cast<float>(pattern_phase) - pf.spiral_pitch * log_ratio)),
+                       0.0f)
+            : 1.0f;
+    const float shear_cos = std::cos(static_cast<float>(shear_phase));
+    const float shear_sin = std::sin(static_cast<float>(shear_phase));
+    const int octaves = std::clamp(static_cast<int>(pf.turbulence_octaves + 0.5f), 1, 6);
+    const uint32_t seed = static_cast<uint32_t>(std::max(pf.noise_seed, 0.0f));
+    const float detail = static_cast<float>(std::clamp(detail_scale, 0.0, 1.0));
+    const float turbulence_noise = fractal_noise(log_ratio * pf.turbulence_scale * pf.radial_stretch,
+                                                 shear_cos * pf.turbulence_scale, shear_sin * pf.turbulence_scale,
+                                                 octaves, seed);
+    const float turbulence = std::exp(1.5f * pf.turbulence_amplitude * detail * turbulence_noise);
+    const float grain_noise = value_noise(log_ratio * pf.grain_scale * pf.radial_stretch, shear_cos * pf.grain_scale,
+                                          shear_sin * pf.grain_scale, seed + 104729U);
+    const float grain = std::max(1.0f + pf.grain_amplitude * detail * grain_noise, 0.0f);
+    const float flux = std::clamp(ring * spiral * turbulence * grain, 0.0f, 8.0f);
+    return std::clamp(envelope * 0.95 * static_cast<double>(pf.opacity_scale) *
+                          std::clamp(sqrt(min(static_cast<double>(flux), 1.0)), 0.15, 1.0),
+                      0.0, 0.98);
+}
+
+// One pixel through the renderer's tracer. y holds the initial KS state (p_t = -1).
+static RayOut trace_render(const RenderCfg& c, double phi_obs, const double y0[8]) {
+    RayOut o;
+    double y[8];
+    std::memcpy(y, y0, sizeof(y));
+    const double lz = y[1] * y[6] - y[2] * y[5];
+    double point[3];
+    bl_point(y + 1, c.a, phi_obs, point);
+    double throughput = 1.0, redshift_rec = 1.0;
+    uint32_t status = 0, iters = 0, flags = 0;
+    const double mid_plane = kPi * 0.5;
+    bool ended = false;
+    for (uint32_t step = 0; step < c.max_steps && throughput > 0.01; ++step) {
+        iters = step + 1;
+        if (point[0] <= c.rh * 1.0001) {
+            status = RT_CLS_CAPTURED;
+            throughput = 0.0;
+            ended = true;
+            break;
+        }
+        if (point[0] >= c.r_esc) {
+            status = RT_CLS_ESCAPED;
+            ended = true;
+            break;
+        }
+        const double r_scale = max(point[0] - c.rh, 0.02 * c.m);
+        const double far = 1.0 + (c.far_scale - 1.0) * std::clamp((point[0] - 20.0 * c.rh) / (80.0 * c.rh), 0.0, 1.0);
+        const double dt_mag = std::clamp(c.kappa * sqrt(point[0] * r_scale) * far, c.h_min, c.h_max * c.far_scale);
+        const double prev[3] = {point[0], point[1], point[2]};
+        render_rk4(y, -dt_mag, c.m, c.a);
+        if (!std::isfinite(y[1]) || !std::isfinite(y[2]) || !std::isfinite(y[3]) || !std::isfinite(y[5]) ||
+            !std::isfinite(y[6]) || !std::isfinite(y[7])) {
+            status = RT_CLS_CAPTURED;
+            flags |= RT_FLAG_NONFINITE;
+            throughput = 0.0;
+            ended = true;
+            break;
+        }
+        bl_point(y + 1, c.a, prev[2], point);
+        if (c.disk_on && (prev[1] - mid_plane) * (point[1] - mid_plane) <= 0.0) {
+            const double span = abs(point[1] - prev[1]);
+            const double s_cross = (span > 1e-12) ? std::clamp(abs(prev[1] - mid_plane) / span, 0.0, 1.0) : 0.5;
+            const double r_cross = prev[0] + s_cross * (point[0] - prev[0]);
+            const double phi_cross = prev[2] + s_cross * (point[2] - prev[2]);
+            if (r_cross >= c.disk_in && r_cross <= c.disk_out && r_cross > c.rh * 1.02) {
+                status |= RT_CLS_DISK;
+                redshift_rec = disk_redshift(c.m, c.disk_spin, r_cross, lz, true);
+                const double opacity = disk_opacity(c.prof, c.m, c.disk_spin, c.disk_in, c.disk_out, r_cross, phi_cross,
+                                                    c.time, c.detail);
+                throughput *= (1.0 - opacity);
+            }
+        }
+    }
+    if (!ended && throughput > 0.01) flags |= RT_FLAG_BUDGET;
+    if ((status & (RT_CLS_CAPTURED | RT_CLS_ESCAPED)) == 0U && point[0] < 3.0 * c.rs) throughput = 0.0;
+    if ((status & RT_CLS_ESCAPED) != 0U || throughput > 0.01) status |= RT_CLS_ESCAPED;
+    o.cls = status | flags;
+    std::memcpy(o.y, y, sizeof(y));
+    o.bl[0] = point[0];
+    o.bl[1] = point[1];
+    o.bl[2] = point[2];
+    o.lambda = iters * 0.05;
+    o.g = redshift_rec;
+    o.acc = iters;
+    o.rej = 0;
+    return o;
 }
 
 }  // namespace rto
@@ -782,11 +1154,40 @@ int rto_trace_rays(const double* x_soa, const double* p_soa, size_t n, const rt_
     return RT_OK;
 }
 
+// RT_METHOD_RENDER_RK4 camera frame (single Kerr): the renderer's tracer on
+// every pixel of the camera's row window.
+int rto_trace_render_camera(const rt_camera* cam, const rt_metric* mt, const rt_integrator* ig,
+                            const rt_termination* tm, rt_results* out, int threads) {
+    if (!cam || !mt || !ig || !tm || !out || cam->width == 0 || cam->height == 0 || mt->n_sources >= 2)
+        return RT_ERR_INVALID_ARG;
+    const Spacetime st = make_spacetime(mt, tm);
+    const RenderCfg rc = make_render(cam, mt, ig, tm);
+    const uint32_t first = std::min(cam->row_offset, cam->height);
+    const uint32_t last = cam->row_count ? std::min(first + cam->row_count, cam->height) : cam->height;
+    const size_t W = cam->width;
+    const size_t n = (size_t)(last - first) * W;
+    parallel_rows(last - first, threads, [&](size_t row) {
+        const uint32_t py = first + (uint32_t)row;
+        for (uint32_t px = 0; px < cam->width; ++px) {
+            double y[8], xi;
+            camera_ray(st, cam, px, py, y, &xi);
+            store(out, n, row * W + px, trace_render(rc, cam->observer[3], y));
+        }
+    });
+    return RT_OK;
+}
+
 // Initial state of one pixel (ok = 0 if the observer is inside a horizon).
 int rto_camera_state(const rt_camera* cam, const rt_metric* mt, const rt_termination* tm, uint32_t px, uint32_t py,
                      double y[8], double* xi) {
     const Spacetime st = make_spacetime(mt, tm);
     return camera_ray(st, cam, px, py, y, xi) ? 1 : 0;
+}
+
+// Ray direction (forward, up, right components) of screen point (u, v) for a
+// projection mode (CameraProjector::compute_ray_direction as dispatch_fp64_scalar calls it).
+void rto_screen_direction(uint32_t mode, double u, double v, double fov, double out[3]) {
+    screen_direction(mode, u, v, fov, out);
 }
 
 // Diagnostic counters: accepted, rejected, projections; reset when reset != 0.

@@ -43,6 +43,8 @@ struct BatchPlan {
     std::vector<uint32_t> m_req;         // request index of each superposition entry
     std::vector<TraceParams> m_params;
     std::vector<MultiDev> m_tables;
+    std::vector<uint32_t> r_req;         // request index of each RT_METHOD_RENDER_RK4 entry
+    std::vector<TraceParams> r_params;
     std::vector<size_t> n_rays;          // per request
     bool mixed = false;   // some request uses the mixed precision policy
     bool carter = false;  // some request asks for RT_TERM_CARTER_CAPTURE
@@ -87,6 +89,11 @@ rt_status plan_batch(const rt_request* reqs, size_t n, BatchPlan& plan, bool til
             plan.m_req.push_back(static_cast<uint32_t>(i));
             plan.m_params.push_back(P);
             plan.m_tables.push_back(T);
+        } else if (P.method == RT_METHOD_RENDER_RK4) {
+            // the renderer's tracer: its own per-request kernel
+            fill_camera(&r.camera, P, nullptr, win);
+            plan.r_req.push_back(static_cast<uint32_t>(i));
+            plan.r_params.push_back(P);
         } else {
             fill_camera(&r.camera, P, nullptr, win);
             plan.q_req.push_back(static_cast<uint32_t>(i));
@@ -224,21 +231,32 @@ hipError_t launch_batch(const BatchPlan& plan, const Placed& pl, const LaunchSha
                         hipStream_t s, const Branches& br) {
     hipError_t e = launch_zero_u64(pl.counters, static_cast<uint32_t>(pl.n_zero), s);
     if (e != hipSuccess) return e;
-    if (!plan.q_params.empty() && shape.queue_kind == 2 && br.n > 0) {
+    const bool per_request = !plan.q_params.empty() && shape.queue_kind == 2;
+    const bool fork = br.n > 0 && (per_request || !plan.r_params.empty());
+    size_t next_branch = 0;
+    auto branch = [&]() {
+        const size_t b = (next_branch++) % static_cast<size_t>(br.n + 1);
+        return b == 0 ? s : br.side[b - 1];
+    };
+    if (fork) {
         e = hipEventRecord(br.fork, s);
         for (int i = 0; i < br.n && e == hipSuccess; ++i) e = hipStreamWaitEvent(br.side[i], br.fork, 0);
-        for (size_t j = 0; j < plan.q_params.size() && e == hipSuccess; ++j) {
-            const size_t b = j % static_cast<size_t>(br.n + 1);
-            e = launch_trace_camera(plan.q_params[j], pl.outs[plan.q_req[j]], kPlainShape,
-                                    b == 0 ? s : br.side[b - 1]);
-        }
+        if (e != hipSuccess) return e;
+    }
+    if (per_request) {
+        for (size_t j = 0; j < plan.q_params.size() && e == hipSuccess; ++j)
+            e = launch_trace_camera(plan.q_params[j], pl.outs[plan.q_req[j]], kPlainShape, fork ? branch() : s);
+    } else if (!plan.q_params.empty()) {
+        e = launch_trace_queue(pl.Q, shape, plan.mixed, plan.carter, grid, s);
+    }
+    for (size_t j = 0; j < plan.r_params.size() && e == hipSuccess; ++j)
+        e = launch_trace_render(plan.r_params[j], pl.outs[plan.r_req[j]], kPlainShape, fork ? branch() : s);
+    if (e != hipSuccess) return e;
+    if (fork) {
         for (int i = 0; i < br.n && e == hipSuccess; ++i) {
             e = hipEventRecord(br.join[i], br.side[i]);
             if (e == hipSuccess) e = hipStreamWaitEvent(s, br.join[i], 0);
         }
-        if (e != hipSuccess) return e;
-    } else if (!plan.q_params.empty()) {
-        e = launch_trace_queue(pl.Q, shape, plan.mixed, plan.carter, grid, s);
         if (e != hipSuccess) return e;
     }
     for (size_t j = 0; j < plan.m_params.size(); ++j) {
@@ -267,6 +285,10 @@ std::vector<unsigned char> graph_key(const BatchPlan& plan, const Placed& pl, co
     for (size_t j = 0; j < plan.m_params.size(); ++j) {
         put(&plan.m_params[j], sizeof(TraceParams));
         put(&pl.outs[plan.m_req[j]], sizeof(OutPtrs));
+    }
+    for (size_t j = 0; j < plan.r_params.size(); ++j) {
+        put(&plan.r_params[j], sizeof(TraceParams));
+        put(&pl.outs[plan.r_req[j]], sizeof(OutPtrs));
     }
     return k;
 }

@@ -96,6 +96,22 @@ bool valid_request(const rt_metric* mt, const rt_integrator* ig, const rt_termin
         why = "integrator: unknown precision policy";
         return false;
     }
+    if (ig->method > RT_METHOD_RENDER_RK4) {
+        why = "integrator: unknown method";
+        return false;
+    }
+    if (ig->method == RT_METHOD_RENDER_RK4) {
+        if (ig->precision != RT_PRECISION_FP64 || mt->n_sources >= 2) {
+            why = "integrator: RT_METHOD_RENDER_RK4 traces single Kerr in FP64";
+            return false;
+        }
+        if (!(ig->step_size_factor > 0.0) || !(ig->min_step > 0.0) || !(ig->far_field_step_scale > 0.0) ||
+            !(ig->max_step * ig->far_field_step_scale >= ig->min_step)) {
+            why = "integrator: render steps need step_size_factor > 0, min_step > 0, far_field_step_scale > 0 and "
+                  "max_step * far_field_step_scale >= min_step";
+            return false;
+        }
+    }
     if (!(tm->escape_radius > 0.0) || !(tm->horizon_eps >= 0.0)) {
         why = "termination: need escape_radius > 0 and horizon_eps >= 0";
         return false;
@@ -123,8 +139,15 @@ bool valid_camera(const rt_camera* c, std::string& why) {
         why = "camera: width and height must be > 0";
         return false;
     }
-    if (!(c->fov_y > 0.0 && c->fov_y < 3.14159)) {
-        why = "camera: fov_y must be in (0, pi)";
+    if (c->projection > RT_PROJ_HAMMER_AITOFF) {
+        why = "camera: unknown projection";
+        return false;
+    }
+    // the pinhole-type projections take tan(fov / 2); the others any positive angle
+    const bool tangent = c->projection == RT_PROJ_PINHOLE || c->projection == RT_PROJ_AUTO_ZOOM ||
+                         c->projection == RT_PROJ_PANINI_CYLINDRICAL;
+    if (tangent ? !(c->fov_y > 0.0 && c->fov_y < 3.14159) : !(c->fov_y > 0.0 && c->fov_y <= 4.0 * kPiH)) {
+        why = tangent ? "camera: fov_y must be in (0, pi) for this projection" : "camera: fov_y must be in (0, 4 pi]";
         return false;
     }
     if (!(c->observer[1] > 0.0)) {
@@ -177,6 +200,35 @@ void fill_params(const rt_metric* mt, const rt_integrator* ig, const rt_terminat
     P.precision = ig->precision;
     P.mixed = (!multi && ig->precision == RT_PRECISION_MIXED) ? 1u : 0u;
     P.switch_r = std::max(ig->mixed_switch_radius, 3.0 * P.rh);
+    // RT_METHOD_RENDER_RK4: the render step formula, the exact-path budget
+    // rule of dispatch_fp64_scalar (render/software_compute_engine.hpp:3826-3828)
+    // and the disk profile's opacity parameters
+    P.method = ig->method;
+    P.kappa = ig->step_size_factor;
+    P.step_min = ig->min_step;
+    P.step_max = ig->max_step;
+    P.far_scale = ig->far_field_step_scale;
+    P.rs = 2.0 * P.M;
+    P.compat_steps = std::max(ig->max_steps / 2U, 256U);
+    P.time = tm->time;
+    const rt_disk_profile& d = tm->disk_profile;
+    P.opacity_scale = d.opacity_scale;
+    P.edge_softness = d.edge_softness;
+    P.rotation_speed_scale = d.rotation_speed_scale;
+    P.ring_amplitude = d.ring_amplitude;
+    P.ring_frequency = d.ring_frequency;
+    P.ring_sharpness = d.ring_sharpness;
+    P.spiral_arm_count = d.spiral_arm_count;
+    P.spiral_pitch = d.spiral_pitch;
+    P.spiral_amplitude = d.spiral_amplitude;
+    P.turbulence_amplitude = d.turbulence_amplitude;
+    P.turbulence_scale = d.turbulence_scale;
+    P.turbulence_octaves = d.turbulence_octaves;
+    P.radial_stretch = d.radial_stretch;
+    P.grain_amplitude = d.grain_amplitude;
+    P.grain_scale = d.grain_scale;
+    P.noise_seed = d.noise_seed;
+    if (P.method == RT_METHOD_RENDER_RK4) P.carter = 0u;
 }
 
 // Source table (restating build_multi_source_data, software_compute_engine.hpp:2738-2798):
@@ -282,6 +334,21 @@ void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T, const rt_tile*
     P.mode = 0;
     P.aspect = static_cast<double>(c->width) / static_cast<double>(c->height);
     P.tan_half = std::tan(c->fov_y * 0.5);
+    P.fov = c->fov_y;
+    P.projection = c->projection;
+    // per-frame constants of the projections (observer/camera_projections.hpp:70-196),
+    // with the reference's expressions
+    switch (c->projection) {
+        case RT_PROJ_FISHEYE_STEREOGRAPHIC: {
+            const double max_fov = std::clamp(c->fov_y, 0.1, 260.0 * kPiH / 180.0);
+            P.proj_c = std::tan(max_fov * 0.5 * 0.5);
+            break;
+        }
+        case RT_PROJ_EQUIRECTANGULAR_360: P.proj_c = c->fov_y / (60.0 * kPiH / 180.0); break;
+        case RT_PROJ_FISHEYE_ORTHOGRAPHIC: P.proj_c = std::sin(std::clamp(c->fov_y * 0.5, 0.01, kPiH * 0.499)); break;
+        case RT_PROJ_HAMMER_AITOFF: P.proj_c = (60.0 * kPiH / 180.0) / std::max(c->fov_y, 0.01); break;
+        default: P.proj_c = 0.0; break;
+    }
     for (int i = 0; i < 3; ++i) {
         P.fwd[i] = c->forward[i];
         P.right[i] = c->right[i];
@@ -294,8 +361,18 @@ void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T, const rt_tile*
     P.cth = std::cos(th);
     P.sph = std::sin(ph);
     P.cph = std::cos(ph);
+    P.phi_obs = ph;
+    // turbulence anti-aliasing factor of the frame (dispatch_fp64_scalar,
+    // render/software_compute_engine.hpp:3834-3836)
+    {
+        const double r_obs_frame = std::max(c->observer[1], P.rh * 1.02);
+        const double angular_pixel_size = c->fov_y / std::max(static_cast<double>(c->width), 1.0);
+        const double bh_angular_diameter = (2.0 * P.rh) / r_obs_frame;
+        P.detail = std::clamp(bh_angular_diameter / std::max(angular_pixel_size * 24.0, 1e-9), 0.0, 1.0);
+    }
     P.hot_x0 = P.hot_y0 = P.hot_w = P.hot_h = 0;
-    if (P.n_src < 2) hot_tiles(P, r_obs);
+    // hot tiles from the pinhole mapping; other projections trace row by row
+    if (P.n_src < 2 && (c->projection == RT_PROJ_PINHOLE || c->projection == RT_PROJ_AUTO_ZOOM)) hot_tiles(P, r_obs);
     if (P.n_src >= 2 && T) {
         P.X0[0] = r_obs * P.sth * P.cph;
         P.X0[1] = r_obs * P.sth * P.sph;
@@ -502,6 +579,8 @@ rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, 
         }
     }
     if (n == 0) return fail(RT_ERR_INVALID_ARG, "no rays to trace");
+    if (P.method == RT_METHOD_RENDER_RK4 && !cam)
+        return fail(RT_ERR_INVALID_ARG, "RT_METHOD_RENDER_RK4 traces cameras, not explicit rays");
     const bool inputs = (cam == nullptr);
     rt_status st = ctx->arena.ensure(results_bytes(n, inputs));
     if (st != RT_OK) return st;
@@ -525,6 +604,8 @@ rt_status trace_one(rt_context* ctx, const rt_camera* cam, const double* x_soa, 
     RT_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (multi)
         RT_HIP(launch_trace_multi(P, O, Td, counters + 8, kMultiShape, ctx->stream));
+    else if (P.method == RT_METHOD_RENDER_RK4)
+        RT_HIP(launch_trace_render(P, O, kPlainShape, ctx->stream));
     else if (cam)
         RT_HIP(launch_trace_camera(P, O, ctx->shape, ctx->stream));
     else
@@ -566,6 +647,11 @@ void rt_default_integrator(rt_integrator* o) {
     o->mixed_switch_radius = 0.0;
     o->max_steps = 20000;
     o->precision = RT_PRECISION_FP64;
+    o->method = RT_METHOD_DP45;
+    o->step_size_factor = 0.5;
+    o->min_step = 0.004;
+    o->max_step = 3.5;
+    o->far_field_step_scale = 7.8;
 }
 
 void rt_default_termination(rt_termination* o) {
@@ -577,6 +663,11 @@ void rt_default_termination(rt_termination* o) {
     o->disk_outer = 20.0;
     o->disk_enabled = 1;
     o->flags = 0;
+    o->time = 0.0;
+    // GpuDiskProfile's defaults (render/gpu_types.hpp:42-77)
+    o->disk_profile = rt_disk_profile{1.0f,  1.0f,  24.0f, 9500.0f, 1200.0f, 0.75f, 1.0f, 2.0f, 0.45f, 1.0f, 0.0f,
+                                      1.0f,  1.0f,  1.0f,  1.0f,    4.0f,    0.35f, 9.0f, 3.0f, 2.0f,  1.2f, 0.25f,
+                                      0.45f, 5.0f,  4.0f,  3.0f,    0.2f,    40.0f, 1.0f, 1.0f, 1.0f,  1.0f};
 }
 
 void rt_default_camera(rt_camera* c, double r_obs, double th, double ph, double fov_y, uint32_t w, uint32_t h) {

@@ -37,6 +37,8 @@ constexpr LaunchShape kMultiShape{64, 2, 64, 0};
 
 // Camera frame (single Kerr): one lane per pixel, lanes of a wave on an 8x8 tile.
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
+// RT_METHOD_RENDER_RK4 camera frame, single Kerr (rt_render.hip).
+hipError_t launch_trace_render(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
 // Explicit initial states (O.in_x / O.in_p), single Kerr.
 hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
 // Superposition (camera or explicit rays per P.mode), table T in device memory.

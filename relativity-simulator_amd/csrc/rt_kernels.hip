@@ -17,6 +17,7 @@
 //                        persistent, idle groups refilled ray by ray
 #include <hip/hip_runtime.h>
 
+#include "rt_camera.hpp"
 #include "rt_device.hpp"
 #include "rt_kernels.h"
 
@@ -29,56 +30,6 @@ struct Lane {
     uint32_t acc, rej, flags, cls;
     uint32_t rhs, proj, f32;
 };
-
-// Camera pixel -> world direction -> the reference's `outward` vector:
-// pinhole ray (camera_projections.hpp:27-41), camera basis, projection onto
-// e_r/e_theta/e_phi and back (software_compute_engine.hpp:3851-3938, 1634-1645).
-__device__ __forceinline__ void camera_outward(const TraceParams& P, uint32_t px, uint32_t py, double o[3]) {
-    const double v = 1.0 - (static_cast<double>(py) + 0.5) / static_cast<double>(P.height) * 2.0;
-    const double u = ((static_cast<double>(px) + 0.5) / static_cast<double>(P.width) * 2.0 - 1.0) * P.aspect;
-    const double n1 = 1.0, n2 = v * P.tan_half, n3 = u * P.tan_half;
-    const double inv_len = 1.0 / sqrt(n1 * n1 + n2 * n2 + n3 * n3);
-    const double l0 = n1 * inv_len, l1 = n2 * inv_len, l2 = n3 * inv_len;
-    const double d0 = l0 * P.fwd[0] + l2 * P.right[0] + l1 * P.up[0];
-    const double d1 = l0 * P.fwd[1] + l2 * P.right[1] + l1 * P.up[1];
-    const double d2 = l0 * P.fwd[2] + l2 * P.right[2] + l1 * P.up[2];
-    const double st = P.sth, ct = P.cth, sp = P.sph, cp = P.cph;
-    const double n_r = d0 * (st * cp) + d1 * (st * sp) + d2 * ct;
-    const double n_th = d0 * (ct * cp) + d1 * (ct * sp) + d2 * (-st);
-    const double n_ph = d0 * (-sp) + d1 * cp;
-    o[0] = n_r * st * cp + n_th * ct * cp - n_ph * sp;
-    o[1] = n_r * st * sp + n_th * ct * sp + n_ph * cp;
-    o[2] = n_r * ct - n_th * st;
-}
-
-// Photon momentum -outward in the observer's Gram–Schmidt frame, normalised
-// to p_t = -1 (kerr_schild_null_integrator.hpp:179-193). The frame is the same
-// for every pixel and comes precomputed in P.
-__device__ __forceinline__ void camera_state_kerr(const TraceParams& P, uint32_t px, uint32_t py, double y[8],
-                                                  double* xi) {
-    double o[3];
-    camera_outward(P, px, py, o);
-    double vv[4] = {P.u0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        vv[c] -= o[0] * P.frame[0][c];
-        vv[c] -= o[1] * P.frame[1][c];
-        vv[c] -= o[2] * P.frame[2][c];
-    }
-    const double two_h = P.two_h_obs;
-    const double lv = vv[0] + P.kobs[0] * vv[1] + P.kobs[1] * vv[2] + P.kobs[2] * vv[3];
-    const double p_t = -vv[0] + two_h * lv;
-    const double inv_e = 1.0 / fmax(-p_t, 1e-12);
-    y[0] = 0.0;
-    y[1] = P.X0[0];
-    y[2] = P.X0[1];
-    y[3] = P.X0[2];
-    y[4] = -1.0;
-    y[5] = (vv[1] + two_h * lv * P.kobs[0]) * inv_e;
-    y[6] = (vv[2] + two_h * lv * P.kobs[1]) * inv_e;
-    y[7] = (vv[3] + two_h * lv * P.kobs[2]) * inv_e;
-    *xi = y[1] * y[6] - y[2] * y[5];
-}
 
 // Render-step heuristic (software_compute_engine.hpp:1677-1683 with the
 // GpuCameraPushConstants defaults) as the initial |h|.
@@ -360,34 +311,6 @@ struct Tally {
         }
     }
 };
-
-// Tile t of a frame (0 <= t < tiles) in hot-first order: the hot rectangle
-// row by row, then the other tiles row by row.
-__device__ __forceinline__ void tile_xy(const TraceParams& P, uint32_t t, uint32_t& tx, uint32_t& ty) {
-    const uint32_t TX = P.tiles_x;
-    const uint32_t n_hot = P.hot_w * P.hot_h;
-    if (t < n_hot) {
-        tx = P.hot_x0 + t % P.hot_w;
-        ty = P.hot_y0 + t / P.hot_w;
-        return;
-    }
-    uint32_t k = t - n_hot;
-    const uint32_t above = P.hot_y0 * TX;
-    if (k >= above) {
-        k -= above;
-        const uint32_t per_row = TX - P.hot_w;
-        const uint32_t mid = P.hot_h * per_row;
-        if (k < mid) {
-            ty = P.hot_y0 + k / per_row;
-            tx = k % per_row;
-            if (tx >= P.hot_x0) tx += P.hot_w;
-            return;
-        }
-        k = (P.hot_y0 + P.hot_h) * TX + (k - mid);
-    }
-    tx = k % TX;
-    ty = k / TX;
-}
 
 // Find the queue segment holding queue index g (req_offset[s] <= g < req_offset[s + 1]).
 __device__ __forceinline__ uint32_t find_segment(const uint64_t* req_offset, uint32_t n_seg, uint64_t g) {

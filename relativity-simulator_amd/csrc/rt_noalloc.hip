@@ -106,8 +106,9 @@ extern "C" rt_status rt_trace_tiles_noalloc(rt_context* ctx, const rt_tile_task*
         if (w.width == 0 || w.height == 0 || w.origin_x >= t.camera.width || w.origin_y >= t.camera.height ||
             w.width > t.camera.width - w.origin_x || w.height > t.camera.height - w.origin_y)
             return fail(RT_ERR_INVALID_ARG, "task " + std::to_string(i) + ": tile empty or not inside the frame");
-        if (t.metric.n_sources >= 2)
-            return fail(RT_ERR_INVALID_ARG, "task " + std::to_string(i) + ": superpositions are not traced by this entry point");
+        if (t.metric.n_sources >= 2 || t.integrator.method != RT_METHOD_DP45)
+            return fail(RT_ERR_INVALID_ARG, "task " + std::to_string(i) +
+                                                ": superpositions and RT_METHOD_RENDER_RK4 are not traced by this entry point");
         fill_params(&t.metric, &t.integrator, &t.termination, P[i]);
         fill_camera(&t.camera, P[i], nullptr, &w);
         const uint64_t n_tiles = static_cast<uint64_t>(P[i].tiles_x) * ((P[i].n_rays / P[i].cols + 7) / 8);

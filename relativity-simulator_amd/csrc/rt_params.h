@@ -39,13 +39,30 @@ struct TraceParams {
     // the slow rays overlap with the bulk of the frame. Tile units, relative
     // to the traced row window; hot_w * hot_h == 0: plain row-major order.
     uint32_t hot_x0, hot_y0, hot_w, hot_h;
+    uint32_t projection;             // RT_PROJ_* mode of the pixel -> ray mapping
+    uint32_t method;                 // rt_method
     double aspect, tan_half;
+    double fov;                      // field of view as given
+    double proj_c;                   // per-frame constant of the projection (see fill_camera)
     double fwd[3], right[3], up[3];
     double sth, cth, sph, cph;       // observer angles
     double X0[3];                    // observer position (KS, or flat for superpositions)
     double frame[3][4];              // Gram–Schmidt spatial frame e_1..e_3 at the observer
     double u0;                       // observer e_0 time component
     double kobs[3], two_h_obs;       // KS null vector and 2H at the observer (single Kerr)
+    // ---- RT_METHOD_RENDER_RK4 (the reference renderer's tracer)
+    double kappa, step_min, step_max, far_scale;  // render step formula
+    double rs;                       // 2 M
+    double phi_obs;                  // observer azimuth (start of the continued BL azimuth)
+    double time;                     // disk pattern time
+    double detail;                   // turbulence anti-aliasing factor of the frame
+    uint32_t compat_steps;           // loop budget: max(max_integration_steps / 2, 256)
+    uint32_t pad_compat;
+    // opacity parameters of the disk profile (GpuDiskProfile fields)
+    float opacity_scale, edge_softness, rotation_speed_scale, ring_amplitude;
+    float ring_frequency, ring_sharpness, spiral_arm_count, spiral_pitch;
+    float spiral_amplitude, turbulence_amplitude, turbulence_scale, turbulence_octaves;
+    float radial_stretch, grain_amplitude, grain_scale, noise_seed;
 };
 
 // One Kerr–Schild source of a superposition, in its local frame.

@@ -42,6 +42,16 @@ RT_PRECISION_FP64 = 0
 RT_PRECISION_MIXED = 1
 RT_TERM_NO_G_CLAMP = 1
 RT_TERM_CARTER_CAPTURE = 2
+RT_METHOD_DP45 = 0
+RT_METHOD_RENDER_RK4 = 1
+RT_PROJ_PINHOLE = 0
+RT_PROJ_AUTO_ZOOM = 1
+RT_PROJ_FISHEYE_STEREOGRAPHIC = 2
+RT_PROJ_EQUIRECTANGULAR_360 = 3
+RT_PROJ_FISHEYE_EQUIDISTANT = 4
+RT_PROJ_FISHEYE_ORTHOGRAPHIC = 5
+RT_PROJ_PANINI_CYLINDRICAL = 6
+RT_PROJ_HAMMER_AITOFF = 7
 
 STATUS_NAMES = {
     0: "RT_OK", 1: "RT_ERR_INVALID_ARG", 2: "RT_ERR_OOM", 3: "RT_ERR_DEVICE", 4: "RT_ERR_TIMEOUT",
@@ -69,18 +79,38 @@ class rt_metric(C.Structure):
 class rt_integrator(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("h0", C.c_double), ("h_min", C.c_double),
                 ("h_max", C.c_double), ("invariant_tol", C.c_double), ("mixed_switch_radius", C.c_double),
-                ("max_steps", C.c_uint32), ("precision", C.c_uint32)]
+                ("max_steps", C.c_uint32), ("precision", C.c_uint32), ("method", C.c_uint32),
+                ("reserved", C.c_uint32), ("step_size_factor", C.c_double), ("min_step", C.c_double),
+                ("max_step", C.c_double), ("far_field_step_scale", C.c_double)]
+
+
+DISK_PROFILE_FIELDS = [
+    "enabled", "inner_radius_scale", "outer_radius_mass_units", "peak_temperature_k", "floor_temperature_k",
+    "temperature_exponent", "zero_torque_strength", "extra_beaming_exponent", "brightness", "color_saturation",
+    "tint_strength", "opacity_scale", "tint_r", "tint_g", "tint_b", "rotation_speed_scale", "ring_amplitude",
+    "ring_frequency", "ring_sharpness", "spiral_arm_count", "spiral_pitch", "spiral_amplitude",
+    "turbulence_amplitude", "turbulence_scale", "turbulence_octaves", "radial_stretch", "grain_amplitude",
+    "grain_scale", "reference_luminance", "noise_seed", "temperature_normalization", "edge_softness"]
+# GpuDiskProfile's defaults (render/gpu_types.hpp:42-77)
+DISK_PROFILE_DEFAULTS = [1.0, 1.0, 24.0, 9500.0, 1200.0, 0.75, 1.0, 2.0, 0.45, 1.0, 0.0, 1.0, 1.0, 1.0, 1.0, 4.0,
+                         0.35, 9.0, 3.0, 2.0, 1.2, 0.25, 0.45, 5.0, 4.0, 3.0, 0.2, 40.0, 1.0, 1.0, 1.0, 1.0]
+
+
+class rt_disk_profile(C.Structure):
+    _fields_ = [(f, C.c_float) for f in DISK_PROFILE_FIELDS]
 
 
 class rt_termination(C.Structure):
     _fields_ = [("horizon_eps", C.c_double), ("escape_radius", C.c_double), ("disk_inner", C.c_double),
-                ("disk_outer", C.c_double), ("disk_enabled", C.c_uint32), ("flags", C.c_uint32)]
+                ("disk_outer", C.c_double), ("disk_enabled", C.c_uint32), ("flags", C.c_uint32),
+                ("time", C.c_double), ("disk_profile", rt_disk_profile)]
 
 
 class rt_camera(C.Structure):
     _fields_ = [("observer", C.c_double * 4), ("forward", C.c_double * 3), ("right", C.c_double * 3),
                 ("up", C.c_double * 3), ("fov_y", C.c_double), ("width", C.c_uint32), ("height", C.c_uint32),
-                ("row_offset", C.c_uint32), ("row_count", C.c_uint32)]
+                ("row_offset", C.c_uint32), ("row_count", C.c_uint32), ("projection", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class rt_pixel(C.Structure):
@@ -142,7 +172,8 @@ class rt_server_config(C.Structure):
 
 # ------------------------------------------------------------ defaults
 def default_integrator(rtol: float = 1e-10, atol: float = 1e-12, precision: int = RT_PRECISION_FP64,
-                       max_steps: int = 20000, switch_radius: float = 0.0) -> rt_integrator:
+                       max_steps: int = 20000, switch_radius: float = 0.0,
+                       method: int = RT_METHOD_DP45) -> rt_integrator:
     """Same values as rt_default_integrator()."""
     ig = rt_integrator()
     ig.rtol, ig.atol, ig.h0, ig.h_min, ig.h_max = rtol, atol, 0.0, 1e-12, 1e13
@@ -150,7 +181,19 @@ def default_integrator(rtol: float = 1e-10, atol: float = 1e-12, precision: int 
     ig.mixed_switch_radius = switch_radius
     ig.max_steps = max_steps
     ig.precision = precision
+    ig.method = method
+    ig.step_size_factor, ig.min_step, ig.max_step, ig.far_field_step_scale = 0.5, 0.004, 3.5, 7.8
     return ig
+
+
+def render_integrator(max_integration_steps: int = 2048) -> rt_integrator:
+    """The reference renderer's tracer (RT_METHOD_RENDER_RK4) with the push
+    constants' default step formula; max_steps is max_integration_steps."""
+    return default_integrator(max_steps=max_integration_steps, method=RT_METHOD_RENDER_RK4)
+
+
+def default_disk_profile() -> rt_disk_profile:
+    return rt_disk_profile(*DISK_PROFILE_DEFAULTS)
 
 
 def default_termination(escape_radius: float = 1000.0, disk_outer: float = 20.0, horizon_eps: float = 1e-6,
@@ -160,11 +203,13 @@ def default_termination(escape_radius: float = 1000.0, disk_outer: float = 20.0,
     t.horizon_eps, t.escape_radius, t.disk_inner, t.disk_outer = horizon_eps, escape_radius, 0.0, disk_outer
     t.disk_enabled = 1 if disk_enabled else 0
     t.flags = flags
+    t.time = 0.0
+    t.disk_profile = default_disk_profile()
     return t
 
 
 def default_camera(r_obs: float, theta_obs: float, phi_obs: float, fov_y: float, width: int, height: int,
-                   row_offset: int = 0, row_count: int = 0) -> rt_camera:
+                   row_offset: int = 0, row_count: int = 0, projection: int = RT_PROJ_PINHOLE) -> rt_camera:
     """Observer looking at the hole: forward = -e_r, right = e_phi, up = -e_theta
     (same as rt_default_camera())."""
     c = rt_camera()
@@ -175,6 +220,7 @@ def default_camera(r_obs: float, theta_obs: float, phi_obs: float, fov_y: float,
     c.up[:] = [-ct * cp, -ct * sp, st]
     c.fov_y = fov_y
     c.width, c.height, c.row_offset, c.row_count = width, height, row_offset, row_count
+    c.projection = projection
     return c
 
 

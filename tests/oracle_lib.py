@@ -73,6 +73,9 @@ def oracle():
         lib.rto_test_root.restype = D
         lib.rto_trace_one.argtypes = [PD, P(rt_metric), P(rt_integrator), P(rt_termination), P(C.c_uint32), PD, PD,
                                       PD, PD, P(C.c_uint32), P(C.c_uint32)]
+        lib.rto_trace_render_camera.argtypes = [P(rt_camera), P(rt_metric), P(rt_integrator), P(rt_termination),
+                                                P(rt_results), C.c_int]
+        lib.rto_screen_direction.argtypes = [C.c_uint32, D, D, D, PD]
         if hasattr(lib, "rto_flop_counts"):
             lib.rto_flop_counts.argtypes = [PD]
         _ORACLE = lib
@@ -104,6 +107,11 @@ def ref():
         lib.ref_ms_table.restype = C.c_int
         lib.ref_ms_rhs.argtypes = [P(rt_metric), PD, PD, PD, PD]
         lib.ref_ms_initial_momentum.argtypes = [P(rt_metric), PD, PD, PD]
+        lib.ref_screen_direction.argtypes = [C.c_uint32, D, D, D, PD]
+        lib.ref_render_frame.argtypes = [P(rt_camera), P(rt_metric), P(rt_integrator), P(rt_termination),
+                                         P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float), P(C.c_float)]
+        lib.ref_pixel_layout.argtypes = [P(C.c_uint64)]
+        lib.ref_pixel_layout.restype = C.c_int
         _REF = lib
     return _REF
 
@@ -169,3 +177,25 @@ def trace_one(y0, metric, integ, term):
                            C.byref(acc), C.byref(rej))
     return {"cls": cls.value, "y": yend, "bl": bl, "lam": lam.value, "g": g.value, "acc": acc.value,
             "rej": rej.value}
+
+
+def trace_render_camera(cam, metric, integ, term, threads: int = 0) -> rtsim.Results:
+    """RT_METHOD_RENDER_RK4 frame (the renderer's tracer) on the oracle."""
+    res = rtsim.Results(rtsim.n_rays_of(cam))
+    st = oracle().rto_trace_render_camera(C.byref(cam), C.byref(metric), C.byref(integ), C.byref(term),
+                                          C.byref(res.struct), threads)
+    assert st == 0
+    return res
+
+
+def ref_render_frame(cam, metric, integ, term):
+    """The reference's dispatch_fp64 pixel records of a Kerr frame (libref):
+    status_flags, iterations_used, redshift, affine_parameter, rgb."""
+    n = rtsim.n_rays_of(cam)
+    out = {"status": np.zeros(n, np.uint32), "iters": np.zeros(n, np.uint32), "redshift": np.zeros(n, np.float32),
+           "affine": np.zeros(n, np.float32), "rgb": np.zeros((n, 3), np.float32)}
+    ref().ref_render_frame(C.byref(cam), C.byref(metric), C.byref(integ), C.byref(term),
+                           out["status"].ctypes.data_as(P(C.c_uint32)), out["iters"].ctypes.data_as(P(C.c_uint32)),
+                           out["redshift"].ctypes.data_as(P(C.c_float)), out["affine"].ctypes.data_as(P(C.c_float)),
+                           out["rgb"].ctypes.data_as(P(C.c_float)))
+    return out

@@ -2,6 +2,7 @@
 include/rt_tracer.h declares; the ctypes mirror has the C structs' layout;
 defaults are the documented ones; argument errors come back as statuses."""
 import ctypes as C
+import json
 import os
 import re
 import subprocess
@@ -32,7 +33,8 @@ def test_every_declared_symbol_is_exported():
 
 
 STRUCTS = ["rt_source", "rt_metric", "rt_integrator", "rt_termination", "rt_camera", "rt_results", "rt_stats",
-           "rt_request", "rt_server_config", "rt_launch_shape", "rt_pixel", "rt_tile", "rt_tile_task"]
+           "rt_request", "rt_server_config", "rt_launch_shape", "rt_pixel", "rt_tile", "rt_tile_task",
+           "rt_disk_profile"]
 
 
 def test_struct_layout_matches_ctypes():
@@ -42,7 +44,11 @@ def test_struct_layout_matches_ctypes():
     for s, f in [("rt_integrator", "max_steps"), ("rt_camera", "width"), ("rt_metric", "src"),
                  ("rt_request", "termination"), ("rt_stats", "block_size"), ("rt_results", "rejected"),
                  ("rt_pixel", "redshift"), ("rt_pixel", "iterations_used"), ("rt_tile_task", "dispatch_timestamp"),
-                 ("rt_tile_task", "camera"), ("rt_tile_task", "metric"), ("rt_tile_task", "termination")]:
+                 ("rt_tile_task", "camera"), ("rt_tile_task", "metric"), ("rt_tile_task", "termination"),
+                 ("rt_camera", "projection"), ("rt_integrator", "method"), ("rt_integrator", "far_field_step_scale"),
+                 ("rt_termination", "time"), ("rt_termination", "disk_profile"), ("rt_disk_profile", "edge_softness"),
+                 ("rt_pixel", "r"), ("rt_pixel", "g"), ("rt_pixel", "b"), ("rt_pixel", "a"),
+                 ("rt_pixel", "affine_parameter"), ("rt_pixel", "status_flags")]:
         src += f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));\n'
     src += "return 0;}\n"
     with tempfile.TemporaryDirectory() as d:
@@ -60,11 +66,16 @@ def test_struct_layout_matches_ctypes():
     assert int(got["rt_request.termination"]) == rtsim.rt_request.termination.offset
     assert int(got["rt_stats.block_size"]) == rtsim.rt_stats.block_size.offset
     assert int(got["rt_results.rejected"]) == rtsim.rt_results.rejected.offset
-    # the reference's GpuPixelOutput (render/gpu_types.hpp:1111-1121): 4 colour floats, redshift at 16,
-    # affine parameter, flags, iterations at 28, 32 bytes
-    assert int(got["rt_pixel"]) == 32 == rtsim.PIXEL_DTYPE.itemsize
-    assert int(got["rt_pixel.redshift"]) == 16 == rtsim.PIXEL_DTYPE.fields["redshift"][1]
-    assert int(got["rt_pixel.iterations_used"]) == 28 == rtsim.PIXEL_DTYPE.fields["iterations_used"][1]
+    # rt_pixel against the reference's GpuPixelOutput (render/gpu_types.hpp:1111-1121): the offsets and
+    # size that offsetof/sizeof give on the reference's struct (tests/golden/render_frames.json)
+    ref_layout = json.load(open(os.path.join(REPO, "tests", "golden", "render_frames.json")))["pixel_layout"]
+    fields = ["r", "g", "b", "a", "redshift", "affine_parameter", "status_flags", "iterations_used"]
+    assert [int(got[f"rt_pixel.{f}"]) for f in fields] + [int(got["rt_pixel"])] == ref_layout
+    assert [rtsim.PIXEL_DTYPE.fields[f][1] for f in fields] + [rtsim.PIXEL_DTYPE.itemsize] == ref_layout
+    for s, f in [("rt_camera", "projection"), ("rt_integrator", "method"), ("rt_integrator", "far_field_step_scale"),
+                 ("rt_termination", "time"), ("rt_termination", "disk_profile"), ("rt_disk_profile", "edge_softness")]:
+        assert int(got[f"{s}.{f}"]) == getattr(getattr(rtsim, s), f).offset, (s, f)
+    assert int(got["rt_disk_profile"]) == 128
     for f in ["dispatch_timestamp", "camera", "metric", "termination"]:
         assert int(got[f"rt_tile_task.{f}"]) == getattr(rtsim.rt_tile_task, f).offset, f
 
@@ -80,13 +91,16 @@ def test_defaults_match_python_mirror():
     lib.rt_default_termination(C.byref(tm))
     pt = rtsim.default_termination()
     for f, _ in rtsim.rt_termination._fields_:
-        assert getattr(tm, f) == getattr(pt, f), f
+        if f == "disk_profile":
+            assert bytes(tm.disk_profile) == bytes(pt.disk_profile), f
+        else:
+            assert getattr(tm, f) == getattr(pt, f), f
     cam = rtsim.rt_camera()
     lib.rt_default_camera(C.byref(cam), 1000.0, 1.0, 0.3, 0.35, 64, 32)
     pc = rtsim.default_camera(1000.0, 1.0, 0.3, 0.35, 64, 32)
     for f in ("observer", "forward", "right", "up"):
         assert np.allclose(list(getattr(cam, f)), list(getattr(pc, f)), rtol=0, atol=1e-15)
-    assert lib.rt_abi_version() == 1
+    assert lib.rt_abi_version() == 2
     assert b"librtsim" in lib.rt_version()
 
 
