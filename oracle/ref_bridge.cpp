@@ -269,7 +269,11 @@ int ref_tile_geometry(uint32_t index, uint32_t w, uint32_t h, uint32_t out[4]) {
 // tetrad_e3, field_of_view_rad, metric_mass, metric_spin, escape_radius,
 // screen_width, screen_height, metric_type, max_integration_steps,
 // body_count, primary_disk.enabled, .inner_radius_scale,
-// .outer_radius_mass_units (offsets from the start of the push constants).
+// .outer_radius_mass_units, projection_mode, time, step_size_factor,
+// min_step_size, max_step_size, far_field_step_scale, render_flags,
+// lod_distance_threshold, lod_reduced_steps, observer_boost (offsets from the
+// start of the push constants), then sizeof(GpuDiskProfile) and the
+// RenderFlags::USE_LOD_SYSTEM bit.
 int ref_tile_layout(uint64_t* out) {
     using Render::GpuCameraPushConstants;
     using Render::GpuDiskProfile;
@@ -285,7 +289,17 @@ int ref_tile_layout(uint64_t* out) {
                           offsetof(GpuCameraPushConstants, max_integration_steps),
                           offsetof(GpuCameraPushConstants, body_count), disk + offsetof(GpuDiskProfile, enabled),
                           disk + offsetof(GpuDiskProfile, inner_radius_scale),
-                          disk + offsetof(GpuDiskProfile, outer_radius_mass_units)};
+                          disk + offsetof(GpuDiskProfile, outer_radius_mass_units),
+                          offsetof(GpuCameraPushConstants, projection_mode), offsetof(GpuCameraPushConstants, time),
+                          offsetof(GpuCameraPushConstants, step_size_factor),
+                          offsetof(GpuCameraPushConstants, min_step_size),
+                          offsetof(GpuCameraPushConstants, max_step_size),
+                          offsetof(GpuCameraPushConstants, far_field_step_scale),
+                          offsetof(GpuCameraPushConstants, render_flags),
+                          offsetof(GpuCameraPushConstants, lod_distance_threshold),
+                          offsetof(GpuCameraPushConstants, lod_reduced_steps),
+                          offsetof(GpuCameraPushConstants, observer_boost), sizeof(GpuDiskProfile),
+                          static_cast<uint64_t>(Render::RenderFlags::USE_LOD_SYSTEM)};
     const int n = static_cast<int>(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n; ++i) out[i] = v[i];
     return n;
@@ -296,9 +310,13 @@ int ref_tile_layout(uint64_t* out) {
 // (task_payloads.cpp:46-112). d: observer[4], e1[4], e2[4], e3[4], fov,
 // mass, spin, escape radius, timestamp; u: tile index, width, height,
 // lod_reduced_steps, flags, compression mode, max steps, metric type, body
-// count; f: disk enabled, inner radius scale, outer radius (M). Returns the
-// bytes written (0 where the reference refuses the task).
-size_t ref_make_tile_task(const double* d, const uint32_t* u, const float* f, uint8_t* out, size_t cap) {
+// count; f: disk enabled, inner radius scale, outer radius (M); d2:
+// min_step_size, max_step_size, step_size_factor, far_field_step_scale,
+// time, lod_distance_threshold, observer_boost[0]; u2: projection_mode,
+// render_flags, the push constants' lod_reduced_steps. Returns the bytes
+// written (0 where the reference refuses the task).
+size_t ref_make_tile_task(const double* d, const uint32_t* u, const float* f, const double* d2, const uint32_t* u2,
+                          uint8_t* out, size_t cap) {
     Distributed::RenderTileTaskPayload task{};
     auto& c = task.camera_constants;
     for (int i = 0; i < 4; ++i) {
@@ -320,6 +338,16 @@ size_t ref_make_tile_task(const double* d, const uint32_t* u, const float* f, ui
     c.primary_disk.enabled = f[0];
     c.primary_disk.inner_radius_scale = f[1];
     c.primary_disk.outer_radius_mass_units = f[2];
+    c.min_step_size = d2[0];
+    c.max_step_size = d2[1];
+    c.step_size_factor = d2[2];
+    c.far_field_step_scale = d2[3];
+    c.time = d2[4];
+    c.lod_distance_threshold = d2[5];
+    c.observer_boost[0] = d2[6];
+    c.projection_mode = u2[0];
+    c.render_flags = u2[1];
+    c.lod_reduced_steps = u2[2];
     const auto h = Distributed::make_render_tile_task_header(u[0], u[1], u[2], static_cast<uint16_t>(u[3]),
                                                              static_cast<uint8_t>(u[4]),
                                                              static_cast<Distributed::TileCompressionMode>(u[5]));
@@ -405,6 +433,45 @@ int ref_pixel_layout(uint64_t* out) {
                           offsetof(GpuPixelOutput, iterations_used), sizeof(GpuPixelOutput)};
     for (int i = 0; i < 9; ++i) out[i] = v[i];
     return 9;
+}
+
+// The reference's tile path on a task's wire bytes: deserialize_render_tile_task
+// (task_payloads.cpp:114-129), the push-constant adjustments TileRenderer::render
+// makes before tracing (src/distributed/tile_renderer.cpp:150-161: interlace
+// and row window off, a non-zero header lod_reduced_steps in, the tile
+// origin and extent), then SoftwareComputeEngine::dispatch_tile
+// (render/software_compute_engine.hpp:905-1022) without bodies. Writes the
+// tile's GpuPixelOutput fields, tile-local row-major; returns the pixel
+// count (0 when the reference refuses the task).
+size_t ref_render_tile_task(const uint8_t* bytes, size_t n, uint32_t* status, uint32_t* iters, float* redshift,
+                            float* affine) {
+    Distributed::RenderTileTaskPayload task{};
+    if (!Distributed::deserialize_render_tile_task(std::span<const uint8_t>(bytes, n), task)) return 0;
+    Render::GpuCameraPushConstants c = task.camera_constants;
+    const auto g = Distributed::resolve_tile_geometry(task.header.tile_index, c.screen_width, c.screen_height);
+    if (!g) return 0;
+    c.interlace_mode = 0U;
+    c.interlace_phase = 0U;
+    c.dispatch_row_offset = 0U;
+    c.dispatch_row_count = 0U;
+    if (task.header.lod_reduced_steps != 0U) c.lod_reduced_steps = task.header.lod_reduced_steps;
+    c.tile_origin_x = g->origin_x;
+    c.tile_origin_y = g->origin_y;
+    c.tile_extent_x = g->width;
+    c.tile_extent_y = g->height;
+    const size_t count = static_cast<size_t>(g->width) * g->height;
+    std::vector<Render::GpuPixelOutput> px(count);
+    const Render::SoftwareComputeEngine::TileRegion region{g->origin_x, g->origin_y, g->width, g->height};
+    if (!Render::SoftwareComputeEngine::dispatch_tile(c, region, std::span<Render::GpuPixelOutput>(px.data(), count),
+                                                      {}, nullptr))
+        return 0;
+    for (size_t i = 0; i < count; ++i) {
+        status[i] = px[i].status_flags;
+        iters[i] = px[i].iterations_used;
+        redshift[i] = px[i].redshift;
+        affine[i] = px[i].affine_parameter;
+    }
+    return count;
 }
 
 // deserialize_render_tile_task (task_payloads.cpp:114-129): 1 when it

@@ -30,8 +30,11 @@ static_assert(kPushOffset + kPushBytes == RT_TILE_TASK_BYTES, "task size");
 
 // GpuCameraPushConstants field offsets.
 constexpr size_t kObserver = 0, kTetradE1 = 64, kTetradE2 = 96, kTetradE3 = 128, kFov = 160, kMass = 168,
-                 kSpin = 176, kEscape = 240, kScreenW = 584, kScreenH = 588, kMetricType = 592, kMaxSteps = 604,
-                 kBodyCount = 660, kPrimaryDisk = 780;
+                 kSpin = 176, kMinStep = 216, kMaxStep = 224, kEscape = 240, kLodThreshold = 312, kFarScale = 520,
+                 kTime = 528, kStepFactor = 576, kScreenW = 584, kScreenH = 588, kMetricType = 592, kMaxSteps = 604,
+                 kRenderFlags = 608, kProjection = 612, kLodSteps = 620, kBodyCount = 660, kPrimaryDisk = 780,
+                 kObserverBoost = 9192;
+constexpr uint32_t kUseLodSystem = 1u << 9;  // RenderFlags::USE_LOD_SYSTEM (render/gpu_types.hpp:1145)
 // GpuDiskProfile (render/gpu_types.hpp:42-77): enabled, inner_radius_scale,
 // outer_radius_mass_units are its first three floats.
 constexpr size_t kDiskEnabled = 0, kDiskInnerScale = 4, kDiskOuterUnits = 8;
@@ -121,9 +124,11 @@ rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* o
     t.body_count = load<uint32_t>(push, kBodyCount);
     // Values the reference accepts but this tracer's contract does not
     // define: refused here, so that every decoded task can be traced.
-    if (!(fov > 0.0 && fov < 3.14159))
-        return fail(RT_ERR_INVALID_ARG, "tile task: field of view outside (0, pi), which the pinhole camera needs");
     if (!(escape > 0.0)) return fail(RT_ERR_INVALID_ARG, "tile task: escape radius not positive");
+    double boost[3];
+    for (int i = 0; i < 3; ++i) boost[i] = load<double>(push, kObserverBoost + 8 * i);
+    if (boost[0] * boost[0] + boost[1] * boost[1] + boost[2] * boost[2] > 1.0e-18)
+        return fail(RT_ERR_INVALID_ARG, "tile task: observer boost (aberration) is not traced");
 
     // push constants -> request, as dispatch_fp64_scalar reads them
     // (render/software_compute_engine.hpp:3794-3813)
@@ -139,15 +144,40 @@ rt_status rt_decode_tile_task(const void* bytes, size_t n_bytes, rt_tile_task* o
     c.height = H;
     c.row_offset = 0;
     c.row_count = 0;
+    // an unknown projection mode is the pinhole, as compute_ray_direction's default case
+    const uint32_t proj = load<uint32_t>(push, kProjection);
+    c.projection = (proj <= RT_PROJ_HAMMER_AITOFF) ? proj : RT_PROJ_PINHOLE;
     const double m = std::max(mass, 1e-4);
     const double a = (t.metric_type == 1) ? 0.0 : std::clamp(spin, -0.999 * m, 0.999 * m);
+    // step budget under the LOD rule (dispatch_fp64_scalar, software_compute_engine.hpp:3821-3825;
+    // the worker puts a non-zero header lod_reduced_steps into the push
+    // constants, src/distributed/tile_renderer.cpp:155-157), r_+ from the clamped spin
+    uint32_t steps = max_steps;
+    {
+        const double a_pc = std::clamp(spin, -0.999 * m, 0.999 * m);
+        const double rh_pc = (std::abs(a_pc) > 1e-12) ? m + std::sqrt(std::max(m * m - a_pc * a_pc, 0.0)) : 2.0 * m;
+        const double r_frame = std::max(c.observer[1], rh_pc * 1.02);
+        const uint32_t lod_steps = t.lod_reduced_steps != 0 ? t.lod_reduced_steps : load<uint32_t>(push, kLodSteps);
+        const double lod_threshold = load<double>(push, kLodThreshold);
+        if ((load<uint32_t>(push, kRenderFlags) & kUseLodSystem) && lod_threshold > 0.0 && r_frame > lod_threshold)
+            steps = std::min(steps, lod_steps);
+    }
     // observer radius clamped to 1.02 r_+ (dispatch_fp64_scalar, software_compute_engine.hpp:3916)
     c.observer[1] = std::max(c.observer[1], 1.02 * (m + std::sqrt(std::max(m * m - a * a, 0.0))));
+    std::string why;
+    if (!valid_camera(&c, why)) return fail(RT_ERR_INVALID_ARG, "tile task: " + why);
     rt_kerr_metric(&t.metric, m, a);
     rt_default_integrator(&t.integrator);
-    t.integrator.max_steps = max_steps;
+    t.integrator.max_steps = std::max(steps, 1u);
+    t.integrator.min_step = load<double>(push, kMinStep);
+    t.integrator.max_step = load<double>(push, kMaxStep);
+    t.integrator.step_size_factor = load<double>(push, kStepFactor);
+    t.integrator.far_field_step_scale = load<double>(push, kFarScale);
     rt_default_termination(&t.termination);
     t.termination.escape_radius = escape;
+    t.termination.time = load<double>(push, kTime);
+    static_assert(sizeof(rt_disk_profile) == 128, "rt_disk_profile is GpuDiskProfile's 128 bytes");
+    std::memcpy(&t.termination.disk_profile, push + kPrimaryDisk, sizeof(rt_disk_profile));
     const unsigned char* disk = push + kPrimaryDisk;
     const float enabled = load<float>(disk, kDiskEnabled);
     const double inner_scale = std::max(load<float>(disk, kDiskInnerScale), 1.0f);

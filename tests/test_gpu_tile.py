@@ -178,3 +178,32 @@ def test_all_tiles_of_a_frame(ctx, batch_kind):
     parts = split(res, [t.tile.width * t.tile.height for t in batch])
     for t, part in zip(batch, parts):
         same(part, window(frame, W, t.tile), f"tile {t.tile_index}")
+
+
+def test_render_tasks_reproduce_reference_tile_records(ctx):
+    """Decoded tasks traced with RT_METHOD_RENDER_RK4, one by one
+    (rt_trace_tile) and as one batch (rt_trace_tiles): the pixel records
+    (rt_fetch_pixels) equal those of the reference's own tile path on the same
+    wire bytes (tests/golden/tile_tasks.json)."""
+    from test_tile_task import render_task, tile_records
+
+    entries = [e for e in tasks(True) if "reference_tile_records" in e]
+    assert len(entries) >= 6
+    decoded = [render_task(rtsim.decode_tile_task(task_bytes(e))) for e in entries]
+    want_all = []
+    for e, t in zip(entries, decoded):
+        n = ctx.trace_tile(t.camera, t.metric, t.integrator, t.termination, t.tile)
+        pix = ctx.fetch_pixels(n)
+        want = tile_records(e)
+        want_all.append(want)
+        bad = {"status": int((pix["status_flags"] != want["status"]).sum()),
+               "iters": int((pix["iterations_used"] != want["iters"]).sum()),
+               "redshift": int((pix["redshift"] != want["redshift"]).sum()),
+               "affine": int((pix["affine_parameter"] != want["affine"]).sum())}
+        print(f"[render tile] {e['name']}: projection {t.camera.projection} mismatches {bad}", flush=True)
+        assert all(v == 0 for v in bad.values()), (e["name"], bad)
+    n = ctx.trace_tiles(decoded)
+    pix = ctx.fetch_pixels(n)
+    assert np.array_equal(pix["status_flags"], np.concatenate([w["status"] for w in want_all]))
+    assert np.array_equal(pix["iterations_used"], np.concatenate([w["iters"] for w in want_all]))
+    assert np.array_equal(pix["redshift"], np.concatenate([w["redshift"] for w in want_all]))

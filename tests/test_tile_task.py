@@ -85,6 +85,11 @@ def check_decoded(t, e):
     assert tm.horizon_eps == 1e-6 and tm.flags == 0
     assert tm.disk_inner == pytest.approx(e["disk_inner"], rel=1e-14, abs=0.0)
     assert tm.disk_outer == pytest.approx(e["disk_outer"], rel=1e-14)
+    assert c.projection == e["projection"] and tm.time == e["time"]
+    ig = t.integrator
+    assert (ig.min_step, ig.max_step, ig.step_size_factor, ig.far_field_step_scale) == (
+        e["min_step"], e["max_step"], e["step_size_factor"], e["far_field_step_scale"])
+    assert [getattr(tm.disk_profile, f) for f in rtsim.DISK_PROFILE_FIELDS] == e["disk_profile"]
 
 
 @pytest.mark.parametrize("entry", tasks(), ids=lambda e: e["name"])
@@ -103,9 +108,9 @@ def test_decode_ignores_trailing_bytes():
     check_decoded(t, entry["expected"])
 
 
-def test_metric_family_refusals_are_the_only_extra_rule():
+def test_metric_family_and_boost_refusals_are_the_only_extra_rules():
     extra = [t["name"] for t in FIXTURE["tasks"] if t["reference_accepts"] and not t["accepted"]]
-    assert sorted(extra) == ["metric_flat", "metric_reissner_nordstrom"]
+    assert sorted(extra) == ["metric_flat", "metric_reissner_nordstrom", "observer_boost_task"]
 
 
 @pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
@@ -113,8 +118,10 @@ def test_random_mutations_match_reference_verdict():
     """Random byte changes over the header, the timestamp and the first 800
     bytes of the push constants: the decoder accepts exactly what the
     reference's deserialize_render_tile_task accepts, except a metric_type
-    outside 1..3, a field of view outside (0, pi) and an escape radius that
-    is not positive."""
+    outside 1..3, a field of view the task's projection does not accept
+    ((0, pi) for the pinhole-type ones, (0, 4 pi] for the others) and an
+    escape radius that is not positive. (The observer boost lies beyond the
+    mutated bytes.)"""
     ref = O.ref()
     ref.ref_accepts_tile_task.argtypes = [C.c_void_p, C.c_size_t]
     ref.ref_accepts_tile_task.restype = C.c_int
@@ -133,7 +140,10 @@ def test_random_mutations_match_reference_verdict():
         metric_type = int.from_bytes(data[mt_off:mt_off + 4], "little")
         fov = struct.unpack_from("<d", data, push + FIXTURE["layout"]["field_of_view_rad"])[0]
         escape = struct.unpack_from("<d", data, push + FIXTURE["layout"]["escape_radius"])[0]
-        want = want and 1 <= metric_type <= 3 and 0.0 < fov < 3.14159 and escape > 0.0
+        proj = struct.unpack_from("<I", data, push + FIXTURE["layout"]["projection_mode"])[0]
+        proj = proj if proj <= 7 else 0
+        fov_ok = (0.0 < fov < 3.14159) if proj in (0, 1, 6) else (0.0 < fov <= 4.0 * math.pi)
+        want = want and 1 <= metric_type <= 3 and fov_ok and escape > 0.0
         try:
             rtsim.decode_tile_task(data)
             got = True
@@ -165,7 +175,7 @@ def test_changed_fields_decode_to_traceable_tasks():
         t = rtsim.decode_tile_task(changed("kerr_centre_tile", "observer_position", "<d", r, extra=8))
         assert t.camera.observer[1] == pytest.approx(1.02 * r_h, rel=1e-15)
     for fov in (0.0, -0.1, math.pi, 4.0):
-        with pytest.raises(rtsim.RtError, match="field of view"):
+        with pytest.raises(rtsim.RtError, match="fov_y"):
             rtsim.decode_tile_task(changed("kerr_centre_tile", "field_of_view_rad", "<d", fov))
     for esc in (0.0, -10.0):
         with pytest.raises(rtsim.RtError, match="escape radius"):
@@ -176,3 +186,39 @@ def test_changed_fields_decode_to_traceable_tasks():
         t = rtsim.decode_tile_task(changed("kerr_centre_tile", field, "<f", 2.0))
         assert t.termination.disk_enabled == base.termination.disk_enabled
         assert math.isfinite(t.termination.disk_inner) and math.isfinite(t.termination.disk_outer)
+
+
+def tile_records(entry):
+    r = entry["reference_tile_records"]
+
+    def un(key, dt):
+        return np.frombuffer(zlib.decompress(base64.b64decode(r[key])), dtype=dt)
+
+    return {"status": un("status", np.uint8).astype(np.uint32), "iters": un("iters", np.uint16).astype(np.uint32),
+            "redshift": un("redshift", np.float32), "affine": un("affine", np.float32)}
+
+
+def render_task(task):
+    """The decoded task with the renderer's tracer selected."""
+    task.integrator.method = rtsim.RT_METHOD_RENDER_RK4
+    return task
+
+
+@pytest.mark.parametrize("entry", [t for t in tasks(True) if "reference_tile_records" in t], ids=lambda e: e["name"])
+def test_decoded_task_reproduces_reference_tile_records(entry):
+    """A decoded task traced with RT_METHOD_RENDER_RK4 (here by the oracle,
+    whose records equal the GPU kernel's) gives the pixel records of the
+    reference's own tile path (deserialize_render_tile_task, the worker's
+    push-constant adjustments, dispatch_tile): the decode mapping and the
+    renderer's tracer together, bit for bit."""
+    t = render_task(rtsim.decode_tile_task(task_bytes(entry)))
+    g = t.tile
+    cam = rtsim.rt_camera.from_buffer_copy(t.camera)
+    cam.row_offset, cam.row_count = g.origin_y, g.height
+    rows = O.trace_render_camera(cam, t.metric, t.integrator, t.termination)
+    idx = (np.arange(g.height)[:, None] * cam.width + np.arange(g.origin_x, g.origin_x + g.width)[None, :]).ravel()
+    got = {"status": rows.cls[idx] & 0xFF, "iters": rows.steps[idx], "redshift": rows.g[idx].astype(np.float32),
+           "affine": rows.lam[idx].astype(np.float32)}
+    want = tile_records(entry)
+    bad = {k: int((got[k] != want[k]).sum()) for k in want}
+    assert all(v == 0 for v in bad.values()), bad
