@@ -318,6 +318,15 @@ static double escape_radius_of(const Spacetime& st, const double* x) {
 // Trace one ray from y_init (t, x, y, z, p_t, p_x, p_y, p_z). xi is L_z/E of
 // the ray for the single-hole disk redshift (reference: angular_momentum_z of the
 // initial state, software_compute_engine.hpp:1650).
+// Optional record of every crossing of the equatorial plane z = 0 along a
+// single-Kerr ray (test infrastructure: rto_z_crossings).
+struct CrossingLog {
+    static constexpr int kCap = 512;
+    double r[kCap], lambda[kCap];
+    int n = 0;
+};
+static thread_local CrossingLog* g_crossings = nullptr;
+
 RayOut trace_ray(const Spacetime& st, const Integ& ig, const double y_init[8], double xi) {
     RayOut o;
     double y[8], y5[8], k[7][8];
@@ -386,6 +395,15 @@ RayOut trace_ray(const Spacetime& st, const Integ& ig, const double y_init[8], d
                     }
                 }
                 const double z0 = y[3], z1 = y5[3];
+                if (g_crossings && g_crossings->n < CrossingLog::kCap &&
+                    ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0))) {
+                    const double s = illinois_root([&](double ss) { return dens(ss, 3); }, z0, z1);
+                    double x[3];
+                    dens_x(s, x);
+                    g_crossings->r[g_crossings->n] = ks_radius(x[0], x[1], x[2], st.a);
+                    g_crossings->lambda[g_crossings->n] = lambda + s * ah;
+                    ++g_crossings->n;
+                }
                 if (st.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0))) {
                     const double s = illinois_root([&](double ss) { return dens(ss, 3); }, z0, z1);
                     double x[3];
@@ -1175,6 +1193,30 @@ int rto_trace_render_camera(const rt_camera* cam, const rt_metric* mt, const rt_
         }
     });
     return RT_OK;
+}
+
+// The equatorial crossings (KS radius, affine length) of a single-Kerr ray
+// traced with the disk switched off, in order along the ray; returns their
+// number (at most cap). Test infrastructure for the disk-event checks.
+int rto_z_crossings(const double y0[8], const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm,
+                    double* r_out, double* lambda_out, int cap) {
+    rt_termination off = *tm;
+    off.disk_enabled = 0;
+    const Spacetime st = make_spacetime(mt, &off);
+    const Integ in = make_integ(ig);
+    CrossingLog log;
+    g_crossings = &log;
+    double y[8];
+    std::memcpy(y, y0, sizeof(y));
+    const double xi = (y[1] * y[6] - y[2] * y[5]) / (-y[4]);
+    trace_ray(st, in, y, xi);
+    g_crossings = nullptr;
+    const int n = std::min(log.n, cap);
+    for (int i = 0; i < n; ++i) {
+        r_out[i] = log.r[i];
+        lambda_out[i] = log.lambda[i];
+    }
+    return n;
 }
 
 // Initial state of one pixel (ok = 0 if the observer is inside a horizon).

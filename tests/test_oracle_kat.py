@@ -454,3 +454,60 @@ def test_carter_capture_only_ends_captured_rays(a, incl, disk):
     same = ~early
     assert np.array_equal(fast.steps[same], full.steps[same]) and np.array_equal(fast.x[:, same], full.x[:, same])
     assert fast.steps[early].sum() < 0.2 * full.steps[early].sum()
+
+
+def test_event_end_points_with_the_disk_on():
+    """Where each class of ray ends, with the disk on, checked against the same
+    ray traced with the disk off (which records every crossing of z = 0):
+    - a disk hit ends on z = 0 (BL theta = pi/2) at the ray's FIRST crossing
+      whose radius lies in [r_isco, r_out] and above 1.02 r_+, at that
+      crossing's radius and affine length;
+    - an escaped ray never crossed the disk, ends on r = r_esc, moving outward
+      along the integration direction;
+    - a captured ray never crossed the disk and ends on r = r_+ + eps.
+    A disk taken at a later crossing, a crossing outside the radial range, an
+    escape taken at the wrong radius or while moving inward, or a capture
+    short of the horizon all fail these checks."""
+    import ctypes as C
+
+    lib = O.oracle()
+    lib.rto_z_crossings.argtypes = [O.PD, C.POINTER(rtsim.rt_metric), C.POINTER(rtsim.rt_integrator),
+                                    C.POINTER(rtsim.rt_termination), O.PD, O.PD, C.c_int]
+    lib.rto_z_crossings.restype = C.c_int
+    a = 0.9
+    met = rtsim.kerr_metric(1.0, a)
+    ig = rtsim.default_integrator()
+    tm = rtsim.default_termination(escape_radius=400.0, disk_outer=15.0)
+    rh = 1.0 + math.sqrt(1.0 - a * a)
+    r_in, r_out = O.oracle().rto_isco(1.0, a), 15.0
+    cam = rtsim.default_camera(200.0, math.radians(76.0), 0.4, math.radians(9.0), 28, 28)
+    res = O.trace_camera(cam, met, ig, tm)
+    counts = {1: 0, 2: 0, 4: 0}
+    multi_crossing = 0
+    rbuf, lbuf = np.zeros(512), np.zeros(512)
+    for i in range(res.n):
+        ok, y0, _ = O.camera_state(cam, met, tm, i % cam.width, i // cam.width)
+        assert ok
+        k = lib.rto_z_crossings(y0.ctypes.data_as(O.PD), C.byref(met), C.byref(ig), C.byref(tm),
+                                rbuf.ctypes.data_as(O.PD), lbuf.ctypes.data_as(O.PD), 512)
+        inside = [(r, l) for r, l in zip(rbuf[:k], lbuf[:k]) if r_in <= r <= r_out and r > 1.02 * rh]
+        cls = int(res.klass[i])
+        counts[cls] += 1
+        r_end, th_end, lam = res.bl[0, i], res.bl[1, i], res.lam[i]
+        if cls == rtsim.RT_CLS_DISK:
+            assert inside, f"ray {i}: disk hit without a crossing in range"
+            r_c, l_c = inside[0]
+            assert abs(res.x[3, i]) <= 1e-9 * r_end and abs(th_end - math.pi / 2) <= 1e-9
+            assert abs(r_end - r_c) <= 1e-7 * r_c and abs(lam - l_c) <= 1e-7 * l_c, (i, r_end, r_c, lam, l_c)
+            assert r_in <= r_end <= r_out and r_end > 1.02 * rh
+            multi_crossing += len(inside) > 1
+        else:
+            assert not [1 for r, l in inside if l <= lam * (1 + 1e-12)], f"ray {i}: class {cls} crossed the disk"
+            if cls == rtsim.RT_CLS_ESCAPED:
+                assert abs(r_end - 400.0) <= 1e-10 * 400.0
+                d = O.rhs(np.concatenate([res.x[:, i], res.p[:, i]]), met)
+                assert -(res.x[1:, i] @ d[1:4]) > 0.0, f"ray {i}: escaped moving inward"
+            else:
+                assert abs(r_end - (rh + 1e-6)) <= 1e-9, (i, r_end - rh)
+    print(f"classes {counts}, disk rays with more than one in-range crossing: {multi_crossing}")
+    assert min(counts.values()) >= 20 and multi_crossing >= 5
