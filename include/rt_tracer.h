@@ -440,6 +440,26 @@ rt_status rt_trace_tiles(rt_context* ctx, const rt_tile_task* tasks, size_t n_ta
 rt_status rt_trace_tiles_noalloc(rt_context* ctx, const rt_tile_task* tasks, size_t n_tasks,
                                  const rt_results* dev_out, size_t capacity, size_t* n_rays);
 
+/* ------------------------------------------------------- critical curve
+ * The Kerr critical curve (the shadow's edge) in the impact-parameter plane
+ * (alpha, beta) of an observer at inclination theta_obs, following
+ * BardeenKerrShadow (include/relativistic/metrics/bardeen_shadow.hpp:38-162):
+ * rays whose constants (xi = L_z / E, eta = Q / E^2) lie inside it are
+ * captured, outside it escape. Host computations, no device needed. */
+
+/* Each traced pixel's distance (units of M) from its ray's
+ * (alpha, beta) = (-xi / sin theta_obs, sqrt(eta + a^2 cos^2 theta_obs -
+ * xi^2 cot^2 theta_obs)) to the critical curve; out holds n_rays values in
+ * the camera's pixel order. Rays closer than about 1e-6 M may change class
+ * with the rounding of the integration. Single Kerr only. */
+rt_status rt_critical_distance(const rt_camera* cam, const rt_metric* metric, double* out);
+/* n >= 4 points of the curve, as BardeenKerrShadow::compute_boundary_points:
+ * the upper branch over the visible photon-orbit radii, then the lower branch
+ * back; the circle of radius sqrt(27) M when a = 0 or the observer is polar. */
+rt_status rt_shadow_boundary(double mass, double spin, double theta_obs, size_t n, double* alpha, double* beta);
+/* Distance (units of M) of the point (alpha, beta) to the curve; -1 on invalid input. */
+double rt_impact_distance(double mass, double spin, double theta_obs, double alpha, double beta);
+
 /* ------------------------------------------------------- launch shapes */
 typedef struct rt_launch_shape {
     uint32_t block;       /* threads per workgroup: 64, 128, 192 or 256          */

@@ -13,6 +13,7 @@
 #include "relativistic/core/kerr_schild_null_integrator.hpp"
 #include "relativistic/integrators/rk45_adaptive.hpp"
 #include "relativistic/metrics/kerr_schild.hpp"
+#include "relativistic/metrics/bardeen_shadow.hpp"
 #include "relativistic/optics/disk_thermal_profile.hpp"
 #include "relativistic/render/accretion_disk/accretion_disk_model.hpp"
 #include "relativistic/render/software_compute_engine.hpp"
@@ -472,6 +473,18 @@ size_t ref_render_tile_task(const uint8_t* bytes, size_t n, uint32_t* status, ui
         affine[i] = px[i].affine_parameter;
     }
     return count;
+}
+
+// BardeenKerrShadow (metrics/bardeen_shadow.hpp:38-162): n boundary points
+// (compute_boundary_points) and the enclosed area (compute_shadow_area).
+double ref_shadow_boundary(double m, double a, double theta, size_t n, double* alpha, double* beta) {
+    const Metrics::BardeenKerrShadow shadow(m, a, theta);
+    const auto pts = shadow.compute_boundary_points(n);
+    for (size_t i = 0; i < pts.size() && i < n; ++i) {
+        alpha[i] = pts[i].alpha;
+        beta[i] = pts[i].beta;
+    }
+    return shadow.compute_shadow_area(n);
 }
 
 // deserialize_render_tile_task (task_payloads.cpp:114-129): 1 when it

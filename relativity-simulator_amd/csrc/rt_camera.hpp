@@ -18,7 +18,7 @@
 
 namespace rt {
 
-__device__ __forceinline__ void unit3(double n1, double n2, double n3, double nl[3]) {
+__host__ __device__ __forceinline__ void unit3(double n1, double n2, double n3, double nl[3]) {
     const double len = sqrt(n1 * n1 + n2 * n2 + n3 * n3);
     const double inv_len = (len > 0.0) ? (1.0 / len) : 1.0;
     nl[0] = n1 * inv_len;
@@ -28,7 +28,7 @@ __device__ __forceinline__ void unit3(double n1, double n2, double n3, double nl
 
 // (forward, up, right) components of the ray through screen point (u, v).
 // P.projection is wave-uniform, so the switch is a scalar branch.
-__device__ __forceinline__ void screen_direction(const TraceParams& P, double u, double v, double nl[3]) {
+__host__ __device__ __forceinline__ void screen_direction(const TraceParams& P, double u, double v, double nl[3]) {
     constexpr double kPiD = 3.14159265358979323846;
     switch (P.projection) {
         case RT_PROJ_FISHEYE_STEREOGRAPHIC: {  // proj_c = tan(half of the clamped field / 2)
@@ -118,7 +118,7 @@ __device__ __forceinline__ void screen_direction(const TraceParams& P, double u,
 // Camera pixel (px, py) -> the reference's `outward` vector: screen point,
 // ray direction, world direction from the camera basis, projection onto
 // e_r / e_theta / e_phi at the observer and back to Cartesian.
-__device__ __forceinline__ void camera_outward(const TraceParams& P, uint32_t px, uint32_t py, double o[3]) {
+__host__ __device__ __forceinline__ void camera_outward(const TraceParams& P, uint32_t px, uint32_t py, double o[3]) {
     const double v = 1.0 - (static_cast<double>(py) + 0.5) / static_cast<double>(P.height) * 2.0;
     const double xs = (static_cast<double>(px) + 0.5) / static_cast<double>(P.width) * 2.0 - 1.0;
     const bool allsky = P.projection == RT_PROJ_EQUIRECTANGULAR_360 || P.projection == RT_PROJ_HAMMER_AITOFF;
@@ -140,7 +140,7 @@ __device__ __forceinline__ void camera_outward(const TraceParams& P, uint32_t px
 // Photon momentum -outward in the observer's Gram–Schmidt frame, normalised
 // to p_t = -1 (kerr_schild_null_integrator.hpp:179-193). The frame is the same
 // for every pixel and comes precomputed in P.
-__device__ __forceinline__ void camera_state_kerr(const TraceParams& P, uint32_t px, uint32_t py, double y[8],
+__host__ __device__ __forceinline__ void camera_state_kerr(const TraceParams& P, uint32_t px, uint32_t py, double y[8],
                                                   double* xi) {
     double o[3];
     camera_outward(P, px, py, o);

@@ -306,7 +306,8 @@ EXPORTS = [
     "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
     "rt_fetch_pixels", "rt_tile_geometry", "rt_decode_tile_task", "rt_trace_tile", "rt_trace_tiles",
     "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
-    "rt_server_wait", "rt_server_take", "rt_server_cancel", "rt_trace_tiles_noalloc",
+    "rt_server_wait", "rt_server_take", "rt_server_cancel", "rt_trace_tiles_noalloc", "rt_critical_distance",
+    "rt_shadow_boundary", "rt_impact_distance",
 ]
 
 
@@ -341,6 +342,10 @@ def load(path: Optional[str] = None) -> C.CDLL:
     lib.rt_trace_tiles.argtypes = [C.c_void_p, P(rt_tile_task), C.c_size_t, P(C.c_size_t)]
     lib.rt_trace_tiles_noalloc.argtypes = [C.c_void_p, P(rt_tile_task), C.c_size_t, P(rt_results), C.c_size_t,
                                            P(C.c_size_t)]
+    lib.rt_critical_distance.argtypes = [P(rt_camera), P(rt_metric), P(C.c_double)]
+    lib.rt_shadow_boundary.argtypes = [C.c_double, C.c_double, C.c_double, C.c_size_t, P(C.c_double), P(C.c_double)]
+    lib.rt_impact_distance.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.rt_impact_distance.restype = C.c_double
     lib.rt_trace_batch.argtypes = [C.c_void_p, P(rt_request), C.c_size_t, P(rt_results)]
     lib.rt_get_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
     lib.rt_set_launch_shape.argtypes = [C.c_void_p, P(rt_launch_shape), P(rt_launch_shape)]
@@ -385,6 +390,28 @@ def decode_tile_task(data: bytes) -> rt_tile_task:
     t = rt_tile_task()
     _check(lib, lib.rt_decode_tile_task(buf, len(data), C.byref(t)))
     return t
+
+
+def critical_distance(cam: rt_camera, metric: rt_metric) -> np.ndarray:
+    """Each traced pixel's distance (units of M) to the Kerr critical curve in
+    the impact-parameter plane (rt_critical_distance). Host-only."""
+    lib = load()
+    out = np.zeros(n_rays_of(cam), np.float64)
+    _check(lib, lib.rt_critical_distance(C.byref(cam), C.byref(metric), out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+def shadow_boundary(mass: float, spin: float, theta_obs: float, n: int = 2000):
+    """n points (alpha, beta) of the critical curve (rt_shadow_boundary)."""
+    lib = load()
+    al, be = np.zeros(n), np.zeros(n)
+    _check(lib, lib.rt_shadow_boundary(mass, spin, theta_obs, n, al.ctypes.data_as(C.POINTER(C.c_double)),
+                                       be.ctypes.data_as(C.POINTER(C.c_double))))
+    return al, be
+
+
+def impact_distance(mass: float, spin: float, theta_obs: float, alpha: float, beta: float) -> float:
+    return load().rt_impact_distance(mass, spin, theta_obs, alpha, beta)
 
 
 class Context:
