@@ -50,13 +50,18 @@ R_OBS = 1000.0
 # oracle still counts these values.
 FLOP_RHS = 178
 FLOP_ATTEMPT = 432
+# One source's share of a superposition's right-hand side (offset and
+# rotations into the source's frame, its KS terms, rotations back, sums),
+# from the same operation count.
+FLOP_SOURCE = 244
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per ray of the tracing kernels on MI355X: rocprofv3 FETCH_SIZE +
 # WRITE_SIZE per dispatch, each scaled by what the counter reads for a plain
 # device-to-device copy of known size (x2.000 and x1.000), over the rays of
 # the dispatch (4 x 2048^2 Kerr frames). A camera ray reads nothing and
 # writes 116 B of results.
-HBM_BYTES_PER_RAY = {"k_trace_camera_kerr": 119.86, "k_trace_tiles": 116.55, "k_trace_queue": 116.55}
+HBM_BYTES_PER_RAY = {"k_trace_camera_kerr": 119.86, "k_trace_tiles": 116.55, "k_trace_queue": 116.55,
+                     "k_trace_multi": None}
 METRIC = "rays/s Kerr 2048\u00b2 mixed vs FP64 at \u22641e-9 rel err; p50/p99 latency @QPS"
 
 
@@ -168,34 +173,48 @@ def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
 
 
 def precision_report(side: int, switch_radius: float):
-    """Mixed precision vs FP64 on the GPU: Kerr a = 0.998, inclination 85 deg,
-    side x side camera (4096 by default). Per-output error of the mixed run
-    against the FP64 run, and class agreement."""
-    cam = rtsim.default_camera(R_OBS, math.radians(85.0), 0.0, math.radians(FOV_DEG), side, side)
-    met = rtsim.kerr_metric(1.0, 0.998)
-    tm = rtsim.default_termination()
+    """Mixed precision against FP64 on the precision workload: Kerr a = 0.998,
+    inclination 85 deg, side x side camera (4096: 16.8 M rays), FP32
+    right-hand side beyond switch_radius (0: 3 r_+). The FP64 kernel agrees
+    with the FP64 oracle to ~1e-10 (GPU tests), so the error of the mixed run
+    is measured against the FP64 run on every ray. Reports kernel times and
+    their ratio, per-output error (non-captured rays; max(|ref|, 1)
+    denominator; phi as a wrapped angle), and class agreement overall and by
+    distance to the critical curve (rt_critical_distance: rays within 1e-6 M
+    of it are the north star's exclusion band)."""
+    cam, met, ig64, tm = workloads.precision_request(side)
+    igmx = workloads.precision_request(side, rtsim.RT_PRECISION_MIXED, switch_radius)[2]
     ctx = rtsim.Context(0)
-    rep = {"workload": f"kerr_a0.998_i85_{side}sq", "switch_radius": switch_radius}
-    ref = ctx.trace_camera(cam, met, rtsim.default_integrator(), tm)
+    rep = {"workload": f"kerr_a0.998_i85_robs1000_fov20_{side}sq", "rays": side * side,
+           "switch_radius": switch_radius if switch_radius > 0 else "3 r_+"}
+    ref = ctx.trace_camera(cam, met, ig64, tm)
     rep["fp64_kernel_ms"] = ctx.stats().kernel_ms
-    mx = ctx.trace_camera(cam, met, rtsim.default_integrator(precision=rtsim.RT_PRECISION_MIXED,
-                                                             switch_radius=switch_radius), tm)
+    mx = ctx.trace_camera(cam, met, igmx, tm)
     st = ctx.stats()
+    ctx.close()
     rep["mixed_kernel_ms"] = st.kernel_ms
+    rep["mixed_over_fp64_speed"] = rep["fp64_kernel_ms"] / rep["mixed_kernel_ms"]
     rep["fp32_step_fraction"] = st.fp32_steps / max(1, st.accepted_steps + st.rejected_steps)
     same = mx.klass == ref.klass
     rep["class_agreement"] = float(same.mean())
     rep["class_mismatches"] = int((~same).sum())
+    dist = rtsim.critical_distance(cam, met)
+    bands = {}
+    for lo, hi, name in [(0.0, 1e-6, "within_1e-6"), (1e-6, 1e-3, "1e-6_to_1e-3"), (1e-3, 1e-1, "1e-3_to_0.1"),
+                         (1e-1, np.inf, "beyond_0.1")]:
+        sel = (dist >= lo) & (dist < hi)
+        bands[name] = {"rays": int(sel.sum()), "mismatches": int((~same & sel).sum())}
+    rep["class_by_critical_distance_M"] = bands
+    rep["class_agreement_outside_1e-6"] = float(same[dist >= 1e-6].mean())
     ok = same & (ref.klass != rtsim.RT_CLS_CAPTURED)
     den = lambda v: np.maximum(np.abs(v), 1.0)  # noqa: E731
-    errs = {"x": np.max(np.abs(mx.x[1:, ok] - ref.x[1:, ok]) / den(ref.x[1:, ok]), axis=0),
+    errs = {"ks_position": np.max(np.abs(mx.x[1:, ok] - ref.x[1:, ok]) / den(ref.x[1:, ok]), axis=0),
             "r": np.abs(mx.bl[0, ok] - ref.bl[0, ok]) / den(ref.bl[0, ok]),
             "theta": np.abs(mx.bl[1, ok] - ref.bl[1, ok]) / den(ref.bl[1, ok]),
-            "phi": np.abs(np.angle(np.exp(1j * (mx.bl[2, ok] - ref.bl[2, ok])))) / np.pi,
+            "phi": np.abs(np.remainder(mx.bl[2, ok] - ref.bl[2, ok] + np.pi, 2 * np.pi) - np.pi) / den(ref.bl[2, ok]),
             "g": np.abs(mx.g[ok] - ref.g[ok]) / den(ref.g[ok])}
     rep["rel_err"] = {k: {"p50": float(np.median(v)), "p99": float(np.percentile(v, 99)), "max": float(v.max()),
                           "frac_le_1e-9": float((v <= 1e-9).mean())} for k, v in errs.items()}
-    ctx.close()
     return rep
 
 
@@ -252,6 +271,38 @@ def tile_report(side: int = SIDE, spin: float = 0.9, n_sample: int = 64, seed: i
             "single_tile_centre": len(centre), "single_tile_kernel_ms": pct(dev), "single_tile_host_ms": pct(host)}
 
 
+def launch_flops(st, n_src: int) -> float:
+    """Algorithmic FP64 FLOP of one launch from its counters."""
+    per_rhs = FLOP_RHS if n_src < 2 else n_src * FLOP_SOURCE
+    return st.rhs_evals * per_rhs + (st.accepted_steps + st.rejected_steps) * FLOP_ATTEMPT
+
+
+def superposition_report(side: int = 512, steps: int = 3):
+    """Superposition frames of 2, 4, 8 and 16 holes (workloads.multi_requests:
+    masses U[0.1, 1], positions U[-25, 25]^3 M, spins U[0, 0.95] on random
+    axes; observer 1000 M, inclination 60 deg, FOV 20 deg), each traced by
+    k_trace_multi (one source per lane of a ray's lane group): kernel time,
+    rays/s and algorithmic FP64 rate against the vector peak."""
+    ctx = rtsim.Context(0)
+    rows = []
+    for r in workloads.multi_requests(side):
+        k = r.metric.n_sources
+        n = ctx.trace_camera_resident(r.camera, r.metric, r.integrator, r.termination)  # warm-up
+        ms, flops = [], []
+        for _ in range(steps):
+            ctx.trace_camera_resident(r.camera, r.metric, r.integrator, r.termination)
+            st = ctx.stats()
+            ms.append(st.kernel_ms)
+            flops.append(launch_flops(st, k))
+        t = float(np.median(ms))
+        achieved = float(np.median(flops)) / (t * 1e-3) / 1e12
+        rows.append({"sources": k, "kernel_ms": t, "rays_per_s": n / (t * 1e-3),
+                     "mean_attempts": (st.accepted_steps + st.rejected_steps) / n, "achieved_tflops": achieved,
+                     "frac": achieved / PEAK_FP64_TFLOPS})
+    ctx.close()
+    return {"frames": f"{side}^2, 2/4/8/16 holes", "kernel": "k_trace_multi", "per_frame": rows}
+
+
 def copy_probe(mib: int):
     """Three plain device-to-device copies of a known size: what FETCH_SIZE
     and WRITE_SIZE read for them calibrates the counters against bytes."""
@@ -283,11 +334,15 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tiles", action="store_true", help="skip the 32x32 tile-task report")
-    ap.add_argument("--workload", choices=["frames4", "uneven512"], default="frames4",
-                    help="frames4: the four 2048^2 frames; uneven512: 512 seeded camera requests in one batch")
+    ap.add_argument("--workload", choices=["frames4", "uneven512", "multi"], default="frames4",
+                    help="frames4: the four 2048^2 frames; uneven512: 512 seeded camera requests in one batch; "
+                         "multi: superposition frames of 2, 4, 8, 16 holes (512^2), one launch each")
     ap.add_argument("--precision-report", type=int, default=0, metavar="SIDE",
                     help="print a mixed-vs-FP64 error report at this camera side (e.g. 4096) and exit")
-    ap.add_argument("--switch-radius", type=float, default=100.0)
+    ap.add_argument("--switch-radius", type=float, default=0.0,
+                    help="mixed precision: FP64 right-hand side inside this radius (0: 3 r_+)")
+    ap.add_argument("--precision-side", type=int, default=4096,
+                    help="camera side of the precision object in the default run (0: leave it out)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run three device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -305,6 +360,9 @@ def main() -> int:
     reqs = frame_requests(args.side, prec)
     if args.workload == "uneven512":
         reqs = [(r.camera, r.metric, r.integrator, r.termination) for r in workloads.uneven_requests()]
+    if args.workload == "multi":
+        reqs = [(r.camera, r.metric, r.integrator, r.termination) for r in workloads.multi_requests(512)]
+        args.path = "plain"
     ctx = rtsim.Context(0)
     tuned = None
     if args.workload == "uneven512" and args.path == "batch":
@@ -322,11 +380,11 @@ def main() -> int:
         if args.path == "batch":
             ctx.trace_batch(batch, fetch=False)
             st = ctx.stats()
-            return [(st.rays, st)]
+            return [(st.rays, st, 0)]
         out = []
         for r in reqs:
             n = ctx.trace_camera_resident(*r)
-            out.append((n, ctx.stats()))
+            out.append((n, ctx.stats(), r[1].n_sources))
         return out
 
     for w in range(args.warmup):
@@ -340,9 +398,9 @@ def main() -> int:
     shape = None
     t0 = time.perf_counter()
     for s in range(args.steps):
-        for n, st in one_step():
+        for n, st, n_src in one_step():
             kernel_ms += st.kernel_ms
-            flops += st.rhs_evals * FLOP_RHS + (st.accepted_steps + st.rejected_steps) * FLOP_ATTEMPT
+            flops += launch_flops(st, n_src)
             rays += n
             launches += 1
             acc += st.accepted_steps
@@ -358,6 +416,8 @@ def main() -> int:
     _, qs = ctx.launch_shapes()
     kernel = ("k_trace_camera_kerr" if args.path == "plain" or qs.queue_kind == 2
               else ("k_trace_queue" if qs.queue_kind == 1 else "k_trace_tiles"))
+    if args.workload == "multi":
+        kernel = "k_trace_multi"
     avg_kernel_s = kernel_ms / launches / 1e3
     achieved = flops / launches / avg_kernel_s / 1e12
     out = {
@@ -373,8 +433,9 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64" if prec == rtsim.RT_PRECISION_FP64 else "f64+f32",
         "data": "synthetic (camera rays generated on device)",
-        "config": {"workload": (f"kerr_{args.side}sq_4spins_i60_robs1000_fov20" if args.workload == "frames4"
-                                else "uneven512_seed_c0ffee1234"), "side": args.side,
+        "config": {"workload": {"frames4": f"kerr_{args.side}sq_4spins_i60_robs1000_fov20",
+                                "uneven512": "uneven512_seed_c0ffee1234",
+                                "multi": "superposition_512sq_2_4_8_16_holes"}[args.workload], "side": args.side,
                    "spins": list(SPINS), "precision": args.precision, "rays_per_step": rays // args.steps,
                    "mean_accepted_steps": acc / rays, "mean_rejected_steps": rej / rays,
                    "rhs_evals_per_ray": rhs / rays, "path": args.path, "kernel_ms_per_launch": kernel_ms / launches,
@@ -382,7 +443,8 @@ def main() -> int:
                    "autotuned": tuned, "kernel": kernel},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": HBM_BYTES_PER_RAY[kernel] * rays / launches},
+                     "traffic": (HBM_BYTES_PER_RAY[kernel] * rays / launches
+                                 if HBM_BYTES_PER_RAY[kernel] is not None else None)},
     }
     ctx.close()
     if not args.no_serve:
@@ -391,6 +453,12 @@ def main() -> int:
     if not args.no_tiles:
         print("bench: 32x32 tile tasks", file=sys.stderr, flush=True)
         out["tiles"] = tile_report(args.side)
+    if args.workload == "frames4":
+        print("bench: superposition frames", file=sys.stderr, flush=True)
+        out["superposition"] = superposition_report()
+    if args.precision_side and args.workload == "frames4":
+        print(f"bench: precision, mixed vs FP64 on {args.precision_side}^2", file=sys.stderr, flush=True)
+        out["precision"] = precision_report(args.precision_side, args.switch_radius)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.side, args.cpu_stride, args.cpu_threads)

@@ -1458,7 +1458,31 @@ extern "C" void rto_flop_counts(double out[4]) {
     (void)rto::dp45_next_step<Cnt>(Cnt(20.0), err, true, Cnt(1e-12), Cnt(1e13));
     out[1] = static_cast<double>(g_ops - rhs_ops);  // stage sums + error norm + controller per attempt
     out[2] = static_cast<double>(rhs_ops);          // six RHS evaluations inside an attempt
-    out[3] = 0.0;
+    // one source's share of the superposed RHS (ms_rhs): offset from its
+    // centre, position and momentum into its frame, its KS terms, the
+    // correction and momentum derivative back to the world frame, the sums
+    Cnt c[3] = {3.0, -2.0, 1.0}, fr[3][3] = {{0.6, 0.8, 0.0}, {-0.8, 0.6, 0.0}, {0.0, 0.0, 1.0}};
+    auto rot = [&](const Cnt v[3], Cnt o[3], bool inverse) {
+        for (int i = 0; i < 3; ++i) {
+            o[i] = inverse ? v[0] * fr[0][i] + v[1] * fr[1][i] + v[2] * fr[2][i]
+                           : v[0] * fr[i][0] + v[1] * fr[i][1] + v[2] * fr[i][2];
+        }
+    };
+    g_ops = 0;
+    const Cnt dx[3] = {y[1] - c[0], y[2] - c[1], y[3] - c[2]};
+    Cnt lx[3], lp[3], two_hs, corr[3], dp[3], wc[3], wp[3];
+    rot(dx, lx, false);
+    rot(y + 5, lp, false);
+    rto::ks_terms<Cnt>(lx[0], lx[1], lx[2], y[4], lp[0], lp[1], lp[2], Cnt(0.5), Cnt(0.3), two_hs, corr, dp);
+    rot(corr, wc, true);
+    rot(dp, wp, true);
+    Cnt acc[7];
+    acc[0] = d[0] + two_hs;
+    for (int i = 0; i < 3; ++i) {
+        acc[1 + i] = d[1 + i] + wc[i];
+        acc[4 + i] = d[5 + i] + wp[i];
+    }
+    out[3] = static_cast<double>(g_ops);
 }
 
 // Integrate y0 over an affine length L (sign of L = direction) with the same

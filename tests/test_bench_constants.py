@@ -23,3 +23,4 @@ def test_flop_constants():
     assert per_component == 61  # 46 stage-sum ops + 15 error-norm ops
     assert bench.FLOP_RHS == rhs
     assert bench.FLOP_ATTEMPT == 7 * per_component + controller
+    assert bench.FLOP_SOURCE == out[3]
