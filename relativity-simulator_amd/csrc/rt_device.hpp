@@ -39,15 +39,11 @@ constexpr double d1 = -12715105075.0 / 11282082432.0, d3 = 87487479700.0 / 32700
 constexpr double kPi = 3.14159265358979323846;
 
 // ---------------------------------------------------- fast FP64 math
-// Reciprocal of a positive normal x: v_rcp_f64 (about 2^-29 relative) and two
-// Newton steps, within about 1 ulp (the correctly rounded division sequence
-// costs twice the instructions).
+// Reciprocal of a positive normal x: v_rcp_f64 (about 2^-29 relative) and one
+// Newton step (error squared, then the final rounding): within about 1 ulp.
 __device__ __forceinline__ double rcp64(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
 // sqrt(x) and 1/sqrt(x) of a positive normal x: v_rsq_f64 (about 2^-23
@@ -133,71 +129,108 @@ __device__ __forceinline__ R sqrtR(R x) {
     return (x > R(0)) ? v : R(0);
 }
 
-// The gradient of k.p is evaluated in closed form instead of through the
-// nine-entry Jacobian of k: with u = x p_x + y p_y, w = y p_x - x p_y,
-// rho = r^2 + a^2, q1 = (r u + a w)/rho and q2 = z p_z / r (so k.p = q1 + q2),
+// Per-hole constants of the right-hand side.
+struct KerrConst {
+    double M, a, a2, four_a2, two_M, three_M, m_four_M, two_Ma2;
+};
+__device__ __forceinline__ KerrConst kerr_const(double M, double a, double a2) {
+    KerrConst K;
+    K.M = M;
+    K.a = a;
+    K.a2 = a2;
+    K.four_a2 = 4.0 * a2;
+    K.two_M = 2.0 * M;
+    K.three_M = 3.0 * M;
+    K.m_four_M = -4.0 * M;
+    K.two_Ma2 = 2.0 * M * a2;
+    return K;
+}
+
+// Hamiltonian right-hand side of one hole in its own frame:
+// ths = 2 H s, k (the KS null vector), dp = s^2 grad H + 2 H s grad(k.p), s = k.p - p_t;
+// the position rate is p - 2 H s k.
+//
+// With diff = x^2 + y^2 + z^2 - a^2 and sq = sqrt(diff^2 + 4 a^2 z^2) the KS
+// radius is r^2 = (diff + sq) / 2, and r^4 + a^2 z^2 = r^2 sq. Hence
+//   H = M r^3 / (r^4 + a^2 z^2) = M r / sq,
+//   grad r = (x r / sq, y r / sq, z rho / (r sq)),   rho = r^2 + a^2,
+//   grad H = hb grad r - (0, 0, 2 M a^2 z / (r sq^2)),
+//   hb = M (3 a^2 z^2 - r^4) / (r^2 sq^2) = M (3 sq - 4 r^2) / sq^2,
+// so the field takes the two reciprocal square roots (of sq^2 and of r^2) and
+// one reciprocal (of rho). The gradient of k.p is taken in closed form instead
+// of through the nine-entry Jacobian of k: with u = x p_x + y p_y,
+// q1 = (r u + a (y p_x - x p_y)) / rho = k_x p_x + k_y p_y and q2 = z p_z / r,
 //   grad(k.p) = A + C grad r,  A = ((r p_x - a p_y)/rho, (r p_y + a p_x)/rho, p_z/r),
-//   C = (u - 2 r q1)/rho - q2/r,
-// and grad H = hb grad r + (0, 0, -2 M a^2 r^3 z / den^2). Same function as
-// the reference's expression, about half the operations.
+//   C = (u - 2 r q1)/rho - q2/r.
+// Same function as the reference's expression (core/kerr_schild_null_integrator.hpp:30-114).
+// Generic over the arithmetic type R (double for FP64 steps, float for the
+// far-field steps of the mixed-precision policy); inputs and outputs are FP64.
+// The returned Field carries H = (2 H) / 2 (exact), used only where a caller
+// reads it.
 template <class R>
-__device__ __forceinline__ Field ks_terms(double Xd, double Yd, double Zd, double ptd, double pxd, double pyd,
-                                          double pzd, double Md, double ad, double a2d, double& two_hs,
-                                          double corr[3], double dp[3]) {
+__device__ __forceinline__ Field ks_core(double Xd, double Yd, double Zd, double ptd, double pxd, double pyd,
+                                         double pzd, const KerrConst& K, double& ths_out, double dp[3]) {
     const R X = R(Xd), Y = R(Yd), Z = R(Zd), pt = R(ptd), px = R(pxd), py = R(pyd), pz = R(pzd);
-    const R M = R(Md), a = R(ad), a2 = R(a2d);
+    const R a = R(K.a), a2 = R(K.a2);
     const R z2 = Z * Z;
-    const R diff = X * X + Y * Y + z2 - a2;
-    const R r2 = fmax(R(0.5) * (diff + sqrtR<R>(diff * diff + R(4) * a2 * z2)), R(1e-24));
+    const R diff = fma(X, X, fma(Y, Y, z2 - a2));
+    R inv_sq;
+    const R sq = sqrt_rsqrtR<R>(fma(diff, diff, R(K.four_a2) * z2), inv_sq);
+    const R r2 = fmax(R(0.5) * (diff + sq), R(1e-24));
     R inv_r;
     const R r = sqrt_rsqrtR<R>(r2, inv_r);
-    const R r3 = r2 * r;
-    const R den = r2 * r2 + a2 * z2;
     const R rho = r2 + a2;
-    const R inv_den = rcpR<R>(den);
     const R inv_rho = rcpR<R>(rho);
-    const R H = M * r3 * inv_den;
-    const R u = X * px + Y * py;
-    const R w = Y * px - X * py;
-    const R q1 = (r * u + a * w) * inv_rho;
-    const R q2 = Z * pz * inv_r;
-    const R s = q1 + q2 - pt;
-    const R ths = R(2) * H * s;
-    const R k0 = (r * X + a * Y) * inv_rho;
-    const R k1 = (r * Y - a * X) * inv_rho;
+    const R drc = r * inv_sq;  // (d r / d x) / x
+    const R H2 = R(K.two_M) * drc;
+    const R k0 = fma(r, X, a * Y) * inv_rho;
+    const R k1 = fma(r, Y, -(a * X)) * inv_rho;
     const R k2 = Z * inv_r;
-    two_hs = double(ths);
-    corr[0] = double(-(ths * k0));
-    corr[1] = double(-(ths * k1));
-    corr[2] = double(-(ths * k2));
-    const R drc = r3 * inv_den;
-    const R inv_den2 = inv_den * inv_den;
-    const R hb = M * r2 * (R(3) * a2 * z2 - r2 * r2) * inv_den2;
-    const R C = (u - R(2) * r * q1) * inv_rho - q2 * inv_r;
+    const R q1 = fma(k0, px, k1 * py);
+    const R q2 = k2 * pz;
+    const R s = (q1 + q2) - pt;
+    const R ths = H2 * s;
+    ths_out = double(ths);
+    const R u = fma(X, px, Y * py);
+    const R C = fma(fma(-(r + r), q1, u), inv_rho, -(q2 * inv_r));
+    const R hb = fma(R(K.m_four_M), r2, R(K.three_M) * sq) * (inv_sq * inv_sq);
     const R ss = s * s;
-    const R G = ss * hb + ths * C;
+    const R G = fma(ss, hb, ths * C);
     const R Gd = G * drc;
-    dp[0] = double(Gd * X + ths * ((r * px - a * py) * inv_rho));
-    dp[1] = double(Gd * Y + ths * ((r * py + a * px) * inv_rho));
-    dp[2] = double(G * (r * rho * Z * inv_den) + ths * (pz * inv_r) - ss * (R(2) * M * a2 * r3 * Z * inv_den2));
+    const R tr = ths * inv_rho;
+    dp[0] = double(fma(Gd, X, tr * fma(r, px, -(a * py))));
+    dp[1] = double(fma(Gd, Y, tr * fma(r, py, a * px)));
+    dp[2] = double(inv_r * fma(Z * inv_sq, fma(-(ss * R(K.two_Ma2)), inv_sq, G * rho), ths * pz));
     Field f;
     f.r = double(r);
-    f.H = double(H);
+    f.H = double(R(0.5) * H2);
     f.k0 = double(k0);
     f.k1 = double(k1);
     f.k2 = double(k2);
     return f;
 }
 
+// The same terms as two_hs = 2 H s, corr = -2 H s k (the superposition sums them over holes).
+template <class R>
+__device__ __forceinline__ Field ks_terms(double Xd, double Yd, double Zd, double ptd, double pxd, double pyd,
+                                          double pzd, const KerrConst& K, double& two_hs, double corr[3],
+                                          double dp[3]) {
+    const Field f = ks_core<R>(Xd, Yd, Zd, ptd, pxd, pyd, pzd, K, two_hs, dp);
+    corr[0] = -(two_hs * f.k0);
+    corr[1] = -(two_hs * f.k1);
+    corr[2] = -(two_hs * f.k2);
+    return f;
+}
+
 // Single-hole RHS for y = (t, x, y, z, p_t, p_x, p_y, p_z); d[4] = 0.
 template <class R = double>
-__device__ __forceinline__ Field rhs_kerr(const double y[8], double M, double a, double a2, double d[8]) {
-    double two_hs, corr[3], dpp[3];
-    const Field f = ks_terms<R>(y[1], y[2], y[3], y[4], y[5], y[6], y[7], M, a, a2, two_hs, corr, dpp);
-    d[0] = -y[4] + two_hs;
-    d[1] = y[5] + corr[0];
-    d[2] = y[6] + corr[1];
-    d[3] = y[7] + corr[2];
+__device__ __forceinline__ Field rhs_kerr(const double y[8], const KerrConst& K, double d[8]) {
+    double ths, dpp[3];
+    const Field f = ks_core<R>(y[1], y[2], y[3], y[4], y[5], y[6], y[7], K, ths, dpp);
+    d[0] = ths - y[4];
+    d[1] = fma(-ths, f.k0, y[5]);
+    d[2] = fma(-ths, f.k1, y[6]);
+    d[3] = fma(-ths, f.k2, y[7]);
     d[4] = 0.0;
     d[5] = dpp[0];
     d[6] = dpp[1];

@@ -31,6 +31,11 @@ struct Lane {
     uint32_t rhs, proj, f32;
 };
 
+// Lane-internal flag bit (not exported): the event test found a sign change
+// in the last accepted attempt but no event inside it (a crossing of z = 0
+// off the disk); the attempt is repeated and taken as a plain step.
+constexpr uint32_t kNoEventRetry = 1u << 29;
+
 // Render-step heuristic (software_compute_engine.hpp:1677-1683 with the
 // GpuCameraPushConstants defaults) as the initial |h|.
 __device__ __forceinline__ double initial_step(double r, double rh, double m) {
@@ -42,52 +47,80 @@ __device__ __forceinline__ double initial_step(double r, double rh, double m) {
 
 // Stages k2..k7 of one DP45 attempt from (y, k1, h): the 5th-order solution
 // y5, k7 = f(y5), the error combination ep = sum_{j<7} e_j k_j and, when
-// WithDense, dn = sum_{j<7} d_j k_j for the continuous extension. The same
-// inputs give bit-identical outputs, so the event path recomputes the stages
-// with WithDense instead of keeping dn live on every step.
+// WithDense, dn = sum_{j<7} d_j k_j for the continuous extension.
+// The stage inputs only need x, y, z and the spatial momentum (the RHS reads
+// no t, and p_t is constant): they are fused multiply-add chains on y with the
+// coefficients scaled by h once per stage. y5 keeps the form y + h (sum),
+// whose rounding does not grow with the number of terms.
 template <bool WithDense, class R>
-__device__ __forceinline__ Field dp45_stages(const double y[8], const double k1[8], double h, double M, double a,
-                                             double a2, double y5[8], double ep[8], double k7[8], double dn[8]) {
+__device__ __forceinline__ Field dp45_stages(const double y[8], const double k1[8], double h, const KerrConst& K,
+                                             double y5[8], double ep[8], double k7[8], double dn[8]) {
     double k2[8], k3[8], k4[8], k5[8], k6[8], Y[8];
+    Y[0] = y[0];
+    Y[4] = y[4];
+    {
+        const double c1 = h * dp::a21;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a21 * k1[i]);
-    rhs_kerr<R>(Y, M, a, a2, k2);
+        for (int i = 1; i < 8; ++i)
+            if (i != 4) Y[i] = fma(c1, k1[i], y[i]);
+    }
+    rhs_kerr<R>(Y, K, k2);
+    {
+        const double c1 = h * dp::a31, c2 = h * dp::a32;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a31 * k1[i] + dp::a32 * k2[i]);
-    rhs_kerr<R>(Y, M, a, a2, k3);
+        for (int i = 1; i < 8; ++i)
+            if (i != 4) Y[i] = fma(c2, k2[i], fma(c1, k1[i], y[i]));
+    }
+    rhs_kerr<R>(Y, K, k3);
+    {
+        const double c1 = h * dp::a41, c2 = h * dp::a42, c3 = h * dp::a43;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a41 * k1[i] + dp::a42 * k2[i] + dp::a43 * k3[i]);
-    rhs_kerr<R>(Y, M, a, a2, k4);
+        for (int i = 1; i < 8; ++i)
+            if (i != 4) Y[i] = fma(c3, k3[i], fma(c2, k2[i], fma(c1, k1[i], y[i])));
+    }
+    rhs_kerr<R>(Y, K, k4);
+    {
+        const double c1 = h * dp::a51, c2 = h * dp::a52, c3 = h * dp::a53, c4 = h * dp::a54;
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
-        Y[i] = y[i] + h * (dp::a51 * k1[i] + dp::a52 * k2[i] + dp::a53 * k3[i] + dp::a54 * k4[i]);
-    rhs_kerr<R>(Y, M, a, a2, k5);
+        for (int i = 1; i < 8; ++i)
+            if (i != 4) Y[i] = fma(c4, k4[i], fma(c3, k3[i], fma(c2, k2[i], fma(c1, k1[i], y[i]))));
+    }
+    rhs_kerr<R>(Y, K, k5);
+    {
+        const double c1 = h * dp::a61, c2 = h * dp::a62, c3 = h * dp::a63, c4 = h * dp::a64, c5 = h * dp::a65;
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
-        Y[i] = y[i] + h * (dp::a61 * k1[i] + dp::a62 * k2[i] + dp::a63 * k3[i] + dp::a64 * k4[i] + dp::a65 * k5[i]);
-    rhs_kerr<R>(Y, M, a, a2, k6);
+        for (int i = 1; i < 8; ++i)
+            if (i != 4) Y[i] = fma(c5, k5[i], fma(c4, k4[i], fma(c3, k3[i], fma(c2, k2[i], fma(c1, k1[i], y[i])))));
+    }
+    rhs_kerr<R>(Y, K, k6);
+    y5[4] = y[4];
+    ep[4] = 0.0;
+    if (WithDense) dn[4] = 0.0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
+        if (i == 4) continue;
         y5[i] = y[i] + h * (dp::a71 * k1[i] + dp::a73 * k3[i] + dp::a74 * k4[i] + dp::a75 * k5[i] + dp::a76 * k6[i]);
         ep[i] = dp::e1 * k1[i] + dp::e3 * k3[i] + dp::e4 * k4[i] + dp::e5 * k5[i] + dp::e6 * k6[i];
         if (WithDense) dn[i] = dp::d1 * k1[i] + dp::d3 * k3[i] + dp::d4 * k4[i] + dp::d5 * k5[i] + dp::d6 * k6[i];
     }
-    return rhs_kerr<R>(y5, M, a, a2, k7);
+    return rhs_kerr<R>(y5, K, k7);
 }
 
 // Event location inside an accepted step [y, y5] of size h (single Kerr):
 // horizon r = r_hz, equatorial disk z = 0 with r in [disk_in, disk_out],
-// escape r = r_esc while outgoing; earliest first. Fills the end state and
-// returns the class (0: no event).
-__device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const double y[8], const double k1[8], double h,
-                                                 double r0, double r1, bool hz, bool dk, bool es, double xi, bool f32,
-                                                 double yend[8], double* s_out, double* g_out) {
-    const double M = P.M, a = P.a, a2 = P.a2;
+// escape r = r_esc while outgoing; earliest first. The stages are recomputed
+// with the dense-output sums from (y, k1, h). Fills the end state and returns
+// the class (0: no event).
+__device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const KerrConst& K, const double y[8],
+                                                 const double k1[8], double h, double r0, double r1, bool hz, bool dk,
+                                                 bool es, double xi, bool f32, double yend[8], double* s_out,
+                                                 double* g_out) {
+    const double a2 = P.a2;
     double y5[8], ep[8], k7[8], dn[8];
     if (f32)
-        dp45_stages<true, float>(y, k1, h, M, a, a2, y5, ep, k7, dn);
+        dp45_stages<true, float>(y, k1, h, K, y5, ep, k7, dn);
     else
-        dp45_stages<true, double>(y, k1, h, M, a, a2, y5, ep, k7, dn);
+        dp45_stages<true, double>(y, k1, h, K, y5, ep, k7, dn);
     // dense-output coefficients of the position only; other components are
     // evaluated one at a time at the end (keeps the event path's registers low)
     auto coef = [&](int i) {
@@ -114,7 +147,7 @@ __device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const dou
         if (rc >= P.disk_in && rc <= P.disk_out && rc > P.rh102 && sd < best) {
             best = sd;
             best_cls = RT_CLS_DISK;
-            best_g = disk_redshift(M, P.disk_spin, rc, xi, P.clamp_g != 0);
+            best_g = disk_redshift(P.M, P.disk_spin, rc, xi, P.clamp_g != 0);
         }
     }
     if (es) {
@@ -139,24 +172,36 @@ __device__ __forceinline__ uint32_t locate_event(const TraceParams& P, const dou
     return best_cls;
 }
 
-// Step-size controller of the mixed policy: same law, FP32 power.
-__device__ __forceinline__ double next_step_fast(double ah, double err, bool accepted, double h_min, double h_max) {
-    const float le = __builtin_amdgcn_logf(fmaxf(static_cast<float>(err), 1e-12f));  // log2(err)
-    if (accepted) {
-        const double factor = (err == 0.0) ? 5.0 : static_cast<double>(__builtin_amdgcn_exp2f(-0.2f * le));
-        const double s = fmin(fmax(0.9 * factor, 0.2), 5.0);
-        return fmin(fmax(ah * s, h_min), h_max);
+// Step-size controller for both outcomes of an attempt (one FP32 log2 and
+// exp2 for the power): accepted, 0.9 err^(-1/5) within [0.2, 5] (5 for
+// err = 0) and |h| within [h_min, h_max]; rejected, 0.9 err^(-1/4) within
+// [0.1, 0.9] (integrators/step_controller.hpp:44-78). FP64: the power is
+// refined by one Newton step on y^n e = 1 (about 6e-13 relative); Fast
+// (mixed policy): the FP32 power as it is.
+template <bool Fast>
+__device__ __forceinline__ double next_step_any(double ah, double err, bool acc, double h_min, double h_max) {
+    const double e = fmax(err, 1e-12);
+    const float le = __builtin_amdgcn_logf(static_cast<float>(e));  // log2(e)
+    double yv = static_cast<double>(__builtin_amdgcn_exp2f((acc ? -0.2f : -0.25f) * le));
+    if (!Fast) {
+        const double y2 = yv * yv;
+        const double y4 = y2 * y2;
+        const double yn = acc ? y4 * yv : y4;
+        yv = yv * fma(1.0 - yn * e, acc ? 0.2 : 0.25, 1.0);
     }
-    const double factor = static_cast<double>(__builtin_amdgcn_exp2f(-0.25f * le));
-    return ah * fmin(fmax(0.9 * factor, 0.1), 0.9);
+    const double factor = (acc && err == 0.0) ? 5.0 : yv;
+    const double s = acc ? fmin(fmax(0.9 * factor, 0.2), 5.0) : fmin(fmax(0.9 * factor, 0.1), 0.9);
+    const double nh = ah * s;
+    return acc ? fmin(fmax(nh, h_min), h_max) : nh;
 }
 
 // Start a ray: first RHS (FSAL seed) and initial step.
 template <bool Carter>
-__device__ __forceinline__ void lane_begin(const TraceParams& P, const double y0[8], double xi, Lane& L) {
+__device__ __forceinline__ void lane_begin(const TraceParams& P, const KerrConst& K, const double y0[8], double xi,
+                                           Lane& L) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
-    const Field f0 = rhs_kerr<double>(L.y, P.M, P.a, P.a2, L.k1);
+    const Field f0 = rhs_kerr<double>(L.y, K, L.k1);
     L.r0 = f0.r;
     const double ah = (P.h0 > 0.0) ? P.h0 : initial_step(L.r0, P.rh, P.M);
     L.ah = fmin(fmax(ah, P.h_min), P.h_max);
@@ -175,92 +220,140 @@ __device__ __forceinline__ void lane_begin(const TraceParams& P, const double y0
 // L.y then holds the end state and L.cls its class. Mixed: an attempt that
 // starts beyond switch_r evaluates the RHS in FP32 (state, stage sums, error
 // test, projection and event location stay FP64).
+//
+// An accepted attempt whose end shows an event (r below r_hz, a sign change
+// of z with the disk on, r beyond r_esc moving out) locates it from the state
+// before the attempt; the attempt's own results are not kept live across the
+// location (they would double the registers of the whole loop). When the
+// sign change of z is a crossing off the disk, the lane repeats the attempt,
+// which gives the same step bit for bit, and takes it with the event test off.
+//
+// Every outcome (out of budget, non-finite, event, Carter capture, plain
+// accept or reject) is decided as flags and the lane state is merged once at
+// the end: with a return per outcome the compiler keeps a copy of the whole
+// lane state on each exit path, and those copies run on every attempt. A lane
+// out of budget still computes the attempt (its wave runs it anyway) and
+// discards it.
 template <bool Mixed>
-__device__ __forceinline__ bool lane_step(const TraceParams& P, Lane& L) {
-    const double M = P.M, a = P.a, a2 = P.a2;
-    if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
-        L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
-        L.cls = (L.r0 < P.near_r) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
-        return true;
-    }
+__device__ __forceinline__ bool lane_step(const TraceParams& P, const KerrConst& K, Lane& L) {
     const double ah = L.ah;
+    const bool over = (L.acc + L.rej >= P.max_steps) || (ah < P.h_min);
     const double h = -ah;
     double y5[8], ep[8], k7[8];
     const bool f32 = Mixed && P.mixed && (L.r0 > P.switch_r);
-    const Field f7 = f32 ? dp45_stages<false, float>(L.y, L.k1, h, M, a, a2, y5, ep, k7, nullptr)
-                         : dp45_stages<false, double>(L.y, L.k1, h, M, a, a2, y5, ep, k7, nullptr);
-    L.rhs += 6;
-    L.f32 += f32 ? 1u : 0u;
-    double err = 0.0;
+    Field f7 = f32 ? dp45_stages<false, float>(L.y, L.k1, h, K, y5, ep, k7, nullptr)
+                   : dp45_stages<false, double>(L.y, L.k1, h, K, y5, ep, k7, nullptr);
+    // err = |h| max_i |sum_j e_j k_j,i| / (rtol max(|y_i|, |y5_i|) + atol), p_t excluded.
+    // The seven ratios share one reciprocal: e_j / sc_j = e_j prod_{k != j} sc_k / prod_k sc_k
+    // (prefix and suffix products). The plain form takes over when the products leave the
+    // normal range.
+    double e[7], sc[7];
     bool finite = true;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0, j = 0; i < 8; ++i) {
         finite = finite && isfinite(y5[i]);
         if (i == 4) continue;  // dp_t/dl = 0: no error in p_t
-        const double e = ah * fabs(ep[i] + dp::e7 * k7[i]);
-        const double sc = P.rtol * fmax(fabs(L.y[i]), fabs(y5[i])) + P.atol;
-        err = fmax(err, e * rcp64(sc));
+        e[j] = fabs(fma(dp::e7, k7[i], ep[i]));
+        sc[j] = fma(P.rtol, fmax(fabs(L.y[i]), fabs(y5[i])), P.atol);
+        ++j;
     }
-    if (!finite || !isfinite(err)) {
-        L.flags |= RT_FLAG_NONFINITE;
-        L.cls = RT_CLS_CAPTURED;
-        return true;
-    }
-    const bool fast = Mixed && P.mixed;
-    if (!(err <= 1.0)) {
-        ++L.rej;
-        L.ah = fast ? next_step_fast(ah, err, false, P.h_min, P.h_max) : next_step(ah, err, false, P.h_min, P.h_max);
-        return false;
-    }
-    ++L.acc;
-    const double r1 = f7.r;
-    const double z0 = L.y[3], z1 = y5[3];
-    const bool hz = (r1 <= P.r_hz);
-    const bool dk = P.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0));
-    const bool es = (r1 >= P.r_esc) && (-(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0);
-    if (hz || dk || es) {
-        double s_ev = 1.0, g_ev = 0.0, ye[8];
-        const uint32_t c = locate_event(P, L.y, L.k1, h, L.r0, r1, hz, dk, es, L.xi, f32, ye, &s_ev, &g_ev);
-        if (c != 0) {
-            L.rhs += 6;
-            const Field fe = ks_field(ye[1], ye[2], ye[3], M, a, a2);
-            if (project_pt(ye, fe.H, fe.k0 * ye[5] + fe.k1 * ye[6] + fe.k2 * ye[7], P.inv_tol)) {
-                L.flags |= RT_FLAG_PROJECTED;
-                ++L.proj;
-            }
+    double pre[7], suf[7];
+    pre[1] = sc[0];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) L.y[i] = ye[i];
-            L.lambda += s_ev * ah;
-            L.cls = c;
-            L.g = g_ev;
-            return true;
+    for (int j = 2; j < 7; ++j) pre[j] = pre[j - 1] * sc[j - 1];
+    suf[5] = sc[6];
+#pragma unroll
+    for (int j = 4; j >= 0; --j) suf[j] = suf[j + 1] * sc[j + 1];
+    double num = e[0] * suf[0];
+#pragma unroll
+    for (int j = 1; j < 6; ++j) num = fmax(num, e[j] * (pre[j] * suf[j]));
+    num = fmax(num, e[6] * pre[6]);
+    const double den = pre[6] * sc[6];
+    double err;
+    if (den >= 1e-290 && den <= 1e290 && num <= 1e290) {
+        err = (ah * num) * rcp64(den);
+    } else {
+        err = 0.0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) err = fmax(err, e[j] / sc[j]);
+        err *= ah;
+    }
+    finite = finite && isfinite(err);
+    const bool acc = !over && finite && err <= 1.0;
+    const double r1 = f7.r;
+    bool ended = over || (!over && !finite);
+    bool retry = false;
+    uint32_t cls = RT_CLS_CAPTURED;
+    double dl = ah;
+    uint32_t nrhs = 6;
+    if (acc && !(L.flags & kNoEventRetry)) {
+        const double z0 = L.y[3], z1 = y5[3];
+        const bool hz = (r1 <= P.r_hz);
+        const bool dk = P.disk_on && ((z0 < 0.0 && z1 >= 0.0) || (z0 > 0.0 && z1 <= 0.0));
+        const bool es = (r1 >= P.r_esc) && (-(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0);
+        if (hz || dk || es) {
+            double s_ev = 1.0, g_ev = 0.0;
+            const uint32_t c = locate_event(P, K, L.y, L.k1, h, L.r0, r1, hz, dk, es, L.xi, f32, y5, &s_ev, &g_ev);
+            if (c == 0) {
+                retry = true;
+            } else {
+                ended = true;
+                cls = c;
+                L.g = g_ev;
+                dl = s_ev * ah;
+                nrhs = 12;  // the attempt and the dense recomputation
+                f7 = ks_field(y5[1], y5[2], y5[3], P.M, P.a, P.a2);
+            }
         }
     }
+    if (acc && !retry) {
+        L.flags &= ~kNoEventRetry;
+        ++L.acc;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
-    L.lambda += ah;
-    if (project_pt(L.y, f7.H, f7.k0 * L.y[5] + f7.k1 * L.y[6] + f7.k2 * L.y[7], P.inv_tol)) {
-        L.flags |= RT_FLAG_PROJECTED;
-        ++L.proj;
-        if (f32)
-            rhs_kerr<float>(L.y, M, a, a2, L.k1);
-        else
-            rhs_kerr<double>(L.y, M, a, a2, L.k1);
-        ++L.rhs;
-    } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) L.k1[i] = k7[i];
-    }
-    // RT_TERM_CARTER_CAPTURE: no turning point, moving inward inside the gate
-    if ((L.flags & kCarterNoTurn) && r1 < L.r0 && r1 < P.carter_gate) {
+        for (int i = 0; i < 8; ++i) {
+            L.y[i] = y5[i];
+            L.k1[i] = k7[i];
+        }
+        L.lambda += dl;
+        if (project_pt(L.y, f7.H, f7.k0 * L.y[5] + f7.k1 * L.y[6] + f7.k2 * L.y[7], P.inv_tol)) {
+            L.flags |= RT_FLAG_PROJECTED;
+            ++L.proj;
+            if (!ended) {
+                if (f32)
+                    rhs_kerr<float>(L.y, K, L.k1);
+                else
+                    rhs_kerr<double>(L.y, K, L.k1);
+                ++L.rhs;
+            }
+        }
+        // RT_TERM_CARTER_CAPTURE: no turning point, moving inward inside the gate
+        if (!ended && (L.flags & kCarterNoTurn) && r1 < L.r0 && r1 < P.carter_gate) {
+            ended = true;
+            L.flags |= RT_FLAG_CARTER;
+        }
         L.r0 = r1;
-        L.cls = RT_CLS_CAPTURED;
-        L.flags |= RT_FLAG_CARTER;
-        return true;
     }
-    L.r0 = r1;
-    L.ah = fast ? next_step_fast(ah, err, true, P.h_min, P.h_max) : next_step(ah, err, true, P.h_min, P.h_max);
-    return false;
+    if (over) {
+        L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
+        cls = (L.r0 < P.near_r) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
+    } else if (!finite) {
+        L.flags |= RT_FLAG_NONFINITE;
+    } else if (!acc) {
+        ++L.rej;
+    }
+    if (retry) {
+        L.flags |= kNoEventRetry;
+    } else if (!over) {
+        L.f32 += f32 ? 1u : 0u;
+        L.rhs += nrhs;
+    }
+    if (ended) {
+        L.cls = cls;
+    } else if (!retry) {
+        L.ah = (Mixed && P.mixed) ? next_step_any<true>(ah, err, acc, P.h_min, P.h_max)
+                                  : next_step_any<false>(ah, err, acc, P.h_min, P.h_max);
+    }
+    return ended;
 }
 
 // Redshift per class and the final flag word.
@@ -268,7 +361,7 @@ __device__ __forceinline__ void lane_finish(Lane& L) {
     const uint32_t c = L.cls & RT_CLS_MASK;
     if (c == RT_CLS_ESCAPED) L.g = 1.0 / (-L.y[4]);
     if (c == RT_CLS_CAPTURED) L.g = 0.0;
-    L.cls |= L.flags & ~kCarterNoTurn;
+    L.cls |= L.flags & ~(kCarterNoTurn | kNoEventRetry);
     if (L.f32) L.cls |= RT_FLAG_FP32_STEPS;
 }
 
@@ -340,9 +433,10 @@ __global__ __launch_bounds__(256, Waves) void k_trace_camera_kerr(TraceParams P,
     if (px < P.cols && py < rows) {
         double y0[8], xi;
         camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
+        const KerrConst K = kerr_const(P.M, P.a, P.a2);
         Lane L;
-        lane_begin<Carter>(P, y0, xi, L);
-        while (!lane_step<Mixed>(P, L)) {
+        lane_begin<Carter>(P, K, y0, xi, L);
+        while (!lane_step<Mixed>(P, K, L)) {
         }
         lane_finish(L);
         store_lane(O, static_cast<uint64_t>(py) * P.cols + px, L, P.a, P.a2);
@@ -364,9 +458,10 @@ __global__ __launch_bounds__(256, Waves) void k_trace_rays_kerr(TraceParams P, O
             y0[4 + c] = O.in_p[c * O.n + i];
         }
         const double xi = (y0[1] * y0[6] - y0[2] * y0[5]) / (-y0[4]);
+        const KerrConst K = kerr_const(P.M, P.a, P.a2);
         Lane L;
-        lane_begin<Carter>(P, y0, xi, L);
-        while (!lane_step<Mixed>(P, L)) {
+        lane_begin<Carter>(P, K, y0, xi, L);
+        while (!lane_step<Mixed>(P, K, L)) {
         }
         lane_finish(L);
         store_lane(O, i, L, P.a, P.a2);
@@ -433,7 +528,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
                         xi = (y0[1] * y0[6] - y0[2] * y0[5]) / (-y0[4]);
                     }
                     if (ok) {
-                        lane_begin<Carter>(P, y0, xi, L);
+                        lane_begin<Carter>(P, kerr_const(P.M, P.a, P.a2), y0, xi, L);
                         active = true;
                     }
                 }
@@ -445,7 +540,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
         }
         if (active) {
             const TraceParams& P = Q.reqs[req];
-            if (lane_step<Mixed>(P, L)) {
+            if (lane_step<Mixed>(P, kerr_const(P.M, P.a, P.a2), L)) {
                 lane_finish(L);
                 store_lane(Q.outs[req], ray, L, P.a, P.a2);
                 tally.add(L);
@@ -495,9 +590,10 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
         if (px < P.cols && py < P.n_rays / P.cols) {
             double y0[8], xi = 0.0;
             camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
+            const KerrConst K = kerr_const(P.M, P.a, P.a2);
             Lane L;
-            lane_begin<Carter>(P, y0, xi, L);
-            while (!lane_step<Mixed>(P, L)) {
+            lane_begin<Carter>(P, K, y0, xi, L);
+            while (!lane_step<Mixed>(P, K, L)) {
             }
             lane_finish(L);
             store_lane(outs[req], static_cast<uint64_t>(py) * P.cols + px, L, P.a, P.a2);
@@ -641,7 +737,7 @@ __device__ __forceinline__ void rhs_group(const GroupSrc& gs, const double y[8],
             pz = y[5] * f0 + y[6] * f1 + y[7] * f2;
         }
         double t, c[3], q[3];
-        ks_terms<double>(lx, ly, lz, y[4], px, py, pz, V->m, V->a, V->a2, t, c, q);
+        ks_terms<double>(lx, ly, lz, y[4], px, py, pz, kerr_const(V->m, V->a, V->a2), t, c, q);
         ths += t;
         {
             const double f0 = V->fx[0], f1 = V->fx[1], f2 = V->fx[2];
