@@ -136,12 +136,13 @@ def cpu_baseline(side: int, stride: int, threads: int):
 
 def serve_latency(qps: float, n: int, seed: int = 0xC0FFEE1234):
     """Poisson arrivals at a fixed QPS into the serving loop; latency is
-    submit -> results on the host. Reported overall and per request kind."""
+    submit -> results on the host (every output). Reported overall and per
+    request kind."""
     reqs = workloads.serving_requests(n, seed)
     arrivals = workloads.poisson_arrivals(n, qps, seed + 1)
     lat, kind = [], []
     rays = 0
-    with rtsim.Server(0, hbm_budget_bytes=16 << 30, max_inflight=256) as srv:
+    with rtsim.Server(0, hbm_budget_bytes=16 << 30, max_inflight=256, staging_mib=4096) as srv:
         pending = {}
         t0 = time.perf_counter()
         i = 0
@@ -330,7 +331,9 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--qps", type=float, default=50.0)
+    ap.add_argument("--qps", type=float, default=50.0, help="the latency object's QPS")
+    ap.add_argument("--qps-levels", type=str, default="25,100",
+                    help="further QPS levels reported in latency_levels (comma-separated; empty: none)")
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tiles", action="store_true", help="skip the 32x32 tile-task report")
@@ -450,6 +453,9 @@ def main() -> int:
     if not args.no_serve:
         print(f"bench: serving {args.serve_requests} requests at {args.qps} QPS", file=sys.stderr, flush=True)
         out["latency"] = serve_latency(args.qps, args.serve_requests)
+        levels = [float(q) for q in args.qps_levels.split(",") if q.strip()]
+        if levels:
+            out["latency_levels"] = [out["latency"]] + [serve_latency(q, args.serve_requests) for q in levels]
     if not args.no_tiles:
         print("bench: 32x32 tile tasks", file=sys.stderr, flush=True)
         out["tiles"] = tile_report(args.side)

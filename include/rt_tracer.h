@@ -31,7 +31,10 @@
  *                        as dispatch_fp64 runs it for a Kerr frame
  *   rt_server_*          the windowed worker of GeodesicComputePipeline
  *                        (include/relativistic/render/geodesic_compute_pipeline.hpp:529-728):
- *                        requests, cancellation, completion polling
+ *                        requests, cancellation, completion polling;
+ *                        rt_server_submit_ex deadlines follow the scheduler's
+ *                        in-flight expiry, DistributionScheduler::collect_expired
+ *                        (src/distributed/distribution_scheduler.cpp:389-405)
  */
 #ifndef RT_TRACER_H
 #define RT_TRACER_H
@@ -43,7 +46,7 @@
 extern "C" {
 #endif
 
-#define RT_ABI_VERSION 2u
+#define RT_ABI_VERSION 3u
 #define RT_MAX_SOURCES 16u
 
 /* ------------------------------------------------------------------ status */
@@ -56,7 +59,8 @@ typedef enum rt_status {
     RT_ERR_CANCELLED = 5,   /* request cancelled before completion */
     RT_ERR_NO_DEVICE = 6,   /* no gfx950 device visible */
     RT_ERR_BUSY = 7,        /* admission refused: footprint exceeds the budget */
-    RT_ERR_NOT_FOUND = 8    /* unknown request id */
+    RT_ERR_NOT_FOUND = 8,   /* unknown request id */
+    RT_ERR_EXPIRED = 9      /* request deadline passed before it finished (rt_server_submit_ex) */
 } rt_status;
 
 /* --------------------------------------------- per-ray class and flag bits
@@ -491,23 +495,66 @@ rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample
 rt_status rt_trace_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, rt_results* outs);
 
 /* ------------------------------------------------------------- serving */
+/* A request loop on one device. Requests are admitted in arrival order as
+ * device memory (hbm_budget_bytes), pinned staging (staging_mib), a stream
+ * (four, one per hardware queue) and the device's wave slots allow; a request
+ * that does not fit is passed by later ones (until it has waited 20 ms, when
+ * memory is what it lacks). Each admitted request runs at once, next to those
+ * already on the device, as a persistent kernel sized to the wave slots it was
+ * granted: single-Kerr requests up to their need, superpositions at most half
+ * the device and never the last quarter, so that requests arriving meanwhile
+ * find room. The kernel writes the outputs asked for straight into pinned
+ * staging. Every request must be taken (rt_server_take) or cancelled to free
+ * its staging. */
 typedef struct rt_server_config {
-    uint64_t hbm_budget_bytes;  /* admission budget for ray state + results       */
-    uint32_t max_inflight;      /* requests resident at once                       */
-    uint32_t slots;             /* persistent lanes' work-queue capacity (rays)    */
+    uint64_t hbm_budget_bytes;  /* device memory for results and tables           */
+    uint32_t max_inflight;      /* requests on the device at once                 */
+    uint32_t staging_mib;       /* pinned host staging for finished results, MiB
+                                   (0: 1024), at most hbm_budget_bytes            */
 } rt_server_config;
 
+/* Outputs of a request (rt_submit_options.outputs; rt_results fields). */
+#define RT_OUT_CLS (1u << 0)
+#define RT_OUT_X (1u << 1)
+#define RT_OUT_P (1u << 2)
+#define RT_OUT_BL (1u << 3)
+#define RT_OUT_LAMBDA (1u << 4)
+#define RT_OUT_G (1u << 5)
+#define RT_OUT_STEPS (1u << 6)
+#define RT_OUT_REJECTED (1u << 7)
+#define RT_OUT_ALL 0xFFu
+/* The reference's pixel record (rt_pixel) needs class, steps, g and lambda. */
+#define RT_OUT_PIXEL (RT_OUT_CLS | RT_OUT_LAMBDA | RT_OUT_G | RT_OUT_STEPS)
+
+typedef struct rt_submit_options {
+    uint32_t outputs;     /* RT_OUT_* bits: only these are copied back and taken */
+    uint32_t reserved;    /* 0 */
+    int64_t deadline_ms;  /* > 0: the request ends RT_ERR_EXPIRED if it has not
+                             finished this many ms after submit (a running kernel
+                             completes and its results are dropped); 0: none   */
+} rt_submit_options;
+
 rt_status rt_server_create(int device, const rt_server_config* cfg, rt_server** out);
+/* Cancels pending requests, waits for those on the device, frees the pools. */
 rt_status rt_server_destroy(rt_server* srv);
-/* Admit a request (copied). Refused with RT_ERR_BUSY if its footprint can
- * never fit the budget; otherwise queued until memory frees up. */
+/* Queue a request (copied). Refused with RT_ERR_BUSY if its footprint can
+ * never fit the budgets; otherwise queued until memory frees up. Same as
+ * rt_server_submit_ex with every output and no deadline. */
 rt_status rt_server_submit(rt_server* srv, const rt_request* req, uint64_t* id);
-/* 1 in *done when request id has finished (results ready for rt_server_take). */
+rt_status rt_server_submit_ex(rt_server* srv, const rt_request* req, const rt_submit_options* opt, uint64_t* id);
+/* 1 in *done when request id has finished (done, cancelled, expired or failed). */
 rt_status rt_server_poll(rt_server* srv, uint64_t id, int* done);
-/* Block until done or timeout_ms (< 0: no limit). */
+/* Block until the request has finished or timeout_ms passed (< 0: no limit;
+ * RT_ERR_TIMEOUT). Returns the request's status: RT_OK, RT_ERR_CANCELLED,
+ * RT_ERR_EXPIRED or a device error. */
 rt_status rt_server_wait(rt_server* srv, uint64_t id, int64_t timeout_ms);
-/* Copy the finished results into host arrays and release the request. */
+/* Finished results into host arrays (NULL fields are skipped; asking for an
+ * output not requested at submit is RT_ERR_INVALID_ARG and leaves the
+ * request in place), then forget the request. A cancelled, expired or failed
+ * request returns its status and is forgotten. RT_ERR_BUSY while unfinished. */
 rt_status rt_server_take(rt_server* srv, uint64_t id, rt_results* out);
+/* Pending: removed at once. Running: reported cancelled at once; the kernel
+ * completes and its results are dropped. Done: results dropped. */
 rt_status rt_server_cancel(rt_server* srv, uint64_t id);
 
 #ifdef __cplusplus

@@ -1,5 +1,5 @@
-// rt_batch.cpp — batched frames (rt_trace_batch), launch shapes and the
-// autotuner (rt_*_launch_shape, rt_autotune), and the serving loop (rt_server_*).
+// rt_batch.cpp — batched frames (rt_trace_batch, rt_trace_tiles), launch
+// shapes and the autotuner (rt_*_launch_shape, rt_autotune).
 //
 // A batch is laid out as one work queue: every single-Kerr request becomes a
 // contiguous range of queue indices (8x8 tiles, or the rays of those tiles in
@@ -13,14 +13,11 @@
 
 #include <algorithm>
 #include <chrono>
-#include <condition_variable>
 #include <cstring>
-#include <deque>
 #include <map>
 #include <mutex>
 #include <new>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/rt_tracer.h"
@@ -418,250 +415,6 @@ static rt_status run_batch(rt_context* ctx, const rt_request* reqs, size_t n_req
 
 using namespace rt;
 
-// ===================================================================== server
-namespace {
-
-enum class ReqState { Pending, Running, Done, Cancelled, Failed };
-
-struct ServerReq {
-    rt_request req;
-    size_t n = 0, bytes = 0;
-    ReqState state = ReqState::Pending;
-    rt_status err = RT_OK;
-    bool cancel = false;
-    std::vector<uint32_t> cls, steps, rejected;
-    std::vector<double> x, p, bl, lambda, g;
-};
-
-struct Inflight {
-    std::vector<uint64_t> ids;
-    size_t off = 0, bytes = 0;
-    int stream = 0;
-    hipEvent_t done = nullptr;
-    Placed pl;
-    std::vector<size_t> n_rays;
-};
-
-// First-fit allocator over [0, size) in 256-byte units.
-struct PoolAlloc {
-    std::map<size_t, size_t> free_;  // offset -> bytes
-    void init(size_t size) {
-        free_.clear();
-        free_[0] = size;
-    }
-    bool alloc(size_t bytes, size_t* off) {
-        bytes = al256(bytes);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= bytes) {
-                *off = it->first;
-                const size_t rest = it->second - bytes;
-                const size_t noff = it->first + bytes;
-                free_.erase(it);
-                if (rest) free_[noff] = rest;
-                return true;
-            }
-        }
-        return false;
-    }
-    void release(size_t off, size_t bytes) {
-        bytes = al256(bytes);
-        auto it = free_.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-};
-
-}  // namespace
-
-struct rt_server {
-    int device = 0;
-    rt_server_config cfg{};
-    void* pool = nullptr;
-    size_t pool_bytes = 0;
-    PoolAlloc alloc;
-    std::vector<hipStream_t> streams;
-    std::vector<bool> stream_busy;
-    LaunchShape qshape = kServerShape;
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::thread worker;
-    bool stop = false;
-    uint64_t next_id = 1;
-    std::map<uint64_t, ServerReq> reqs;
-    std::deque<uint64_t> pending;
-    std::vector<Inflight*> inflight;
-    size_t running_reqs = 0;
-
-    static size_t footprint(size_t n_rays, bool multi) {
-        return results_bytes(n_rays, false) + kCounterBytes + 3 * 256 + sizeof(TraceParams) + sizeof(OutPtrs) +
-               (multi ? sizeof(MultiDev) : 0) + 4096;
-    }
-
-    void run();
-    void launch_locked(std::unique_lock<std::mutex>& lk);
-    void complete(Inflight* b, std::unique_lock<std::mutex>& lk);
-};
-
-void rt_server::launch_locked(std::unique_lock<std::mutex>& lk) {
-    // pick a free stream
-    int sidx = -1;
-    for (size_t i = 0; i < streams.size(); ++i)
-        if (!stream_busy[i]) {
-            sidx = static_cast<int>(i);
-            break;
-        }
-    if (sidx < 0 || pending.empty()) return;
-    // admit pending requests FIFO while they fit in one pool block
-    std::vector<uint64_t> take;
-    std::vector<rt_request> batch;
-    size_t bytes = 0;
-    while (!pending.empty() && running_reqs + take.size() < cfg.max_inflight) {
-        const uint64_t id = pending.front();
-        ServerReq& r = reqs[id];
-        if (r.cancel) {
-            pending.pop_front();
-            r.state = ReqState::Cancelled;
-            cv_done.notify_all();
-            continue;
-        }
-        // A superposition request runs as its own batch (on its own stream),
-        // so the long multi-source kernels do not hold back the single-Kerr
-        // requests admitted with them.
-        const bool multi = r.req.metric.n_sources >= 2;
-        if (!take.empty() && (multi || batch.front().metric.n_sources >= 2)) break;
-        size_t off_probe = 0;
-        const size_t want = bytes + r.bytes;
-        if (!alloc.alloc(want, &off_probe)) break;
-        alloc.release(off_probe, want);
-        take.push_back(id);
-        batch.push_back(r.req);
-        bytes = want;
-        pending.pop_front();
-        if (multi) break;
-    }
-    if (take.empty()) return;
-    Inflight* b = new Inflight();
-    b->ids = take;
-    b->stream = sidx;
-    b->bytes = bytes;
-    alloc.alloc(bytes, &b->off);
-    for (uint64_t id : take) reqs[id].state = ReqState::Running;
-    running_reqs += take.size();
-    stream_busy[sidx] = true;
-    lk.unlock();
-    // plan + place + upload + launch outside the lock
-    BatchPlan plan;
-    rt_status st = plan_batch(batch.data(), batch.size(), plan, qshape.queue_kind != 1);
-    hipError_t he = hipSuccess;
-    if (st == RT_OK) {
-        char* base = static_cast<char*>(pool) + b->off;
-        place_batch(plan, base, base + al256(plan.result_bytes), qshape, b->pl);
-        b->n_rays = plan.n_rays;
-        hipStream_t s = streams[sidx];
-        st = upload_batch(plan, b->pl, s);
-        if (st == RT_OK) he = launch_batch(plan, b->pl, qshape, 0, s, Branches{});
-        if (st == RT_OK && he == hipSuccess) he = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
-        if (st == RT_OK && he == hipSuccess) he = hipEventRecord(b->done, s);
-    }
-    lk.lock();
-    if (st != RT_OK || he != hipSuccess) {
-        for (uint64_t id : take) {
-            ServerReq& r = reqs[id];
-            r.state = ReqState::Failed;
-            r.err = (st != RT_OK) ? st : RT_ERR_DEVICE;
-        }
-        running_reqs -= take.size();
-        stream_busy[sidx] = false;
-        alloc.release(b->off, b->bytes);
-        if (b->done) (void)hipEventDestroy(b->done);
-        delete b;
-        cv_done.notify_all();
-        return;
-    }
-    inflight.push_back(b);
-}
-
-void rt_server::complete(Inflight* b, std::unique_lock<std::mutex>& lk) {
-    // copy results out (outside the lock), then release the pool block
-    std::vector<ServerReq*> rs;
-    for (uint64_t id : b->ids) rs.push_back(&reqs[id]);
-    lk.unlock();
-    hipStream_t s = streams[b->stream];
-    rt_status st = RT_OK;
-    for (size_t i = 0; i < rs.size(); ++i) {
-        ServerReq& r = *rs[i];
-        const size_t n = b->n_rays[i];
-        r.cls.resize(n);
-        r.steps.resize(n);
-        r.rejected.resize(n);
-        r.x.resize(4 * n);
-        r.p.resize(4 * n);
-        r.bl.resize(3 * n);
-        r.lambda.resize(n);
-        r.g.resize(n);
-        rt_results o{r.cls.data(), r.x.data(), r.p.data(), r.bl.data(), r.lambda.data(),
-                     r.g.data(),   r.steps.data(), r.rejected.data()};
-        if (st == RT_OK) st = copy_results_to_host(b->pl.outs[i], n, &o, s);
-    }
-    lk.lock();
-    for (ServerReq* r : rs) {
-        if (r->cancel)
-            r->state = ReqState::Cancelled;
-        else if (st != RT_OK) {
-            r->state = ReqState::Failed;
-            r->err = st;
-        } else
-            r->state = ReqState::Done;
-    }
-    running_reqs -= b->ids.size();
-    stream_busy[b->stream] = false;
-    alloc.release(b->off, b->bytes);
-    (void)hipEventDestroy(b->done);
-    delete b;
-    cv_done.notify_all();
-}
-
-void rt_server::run() {
-    (void)hipSetDevice(device);
-    std::unique_lock<std::mutex> lk(mu);
-    while (!stop) {
-        bool progressed = false;
-        for (size_t i = 0; i < inflight.size();) {
-            Inflight* b = inflight[i];
-            if (hipEventQuery(b->done) == hipSuccess) {
-                inflight.erase(inflight.begin() + static_cast<long>(i));
-                complete(b, lk);
-                progressed = true;
-            } else {
-                ++i;
-            }
-        }
-        const size_t before = inflight.size();
-        launch_locked(lk);
-        if (inflight.size() != before) progressed = true;
-        if (!progressed) cv_work.wait_for(lk, std::chrono::microseconds(200));
-    }
-    // drain: wait for every batch still on the device
-    while (!inflight.empty()) {
-        Inflight* b = inflight.back();
-        inflight.pop_back();
-        lk.unlock();
-        (void)hipEventSynchronize(b->done);
-        lk.lock();
-        complete(b, lk);
-    }
-}
-
 extern "C" {
 
 rt_status rt_get_launch_shape(const rt_context* ctx, rt_launch_shape* plain, rt_launch_shape* queue) {
@@ -744,155 +497,6 @@ rt_status rt_autotune(rt_context* ctx, const rt_request* sample, size_t n_sample
     ctx->qshape = win;
     ctx->tuned = true;
     if (best) *best = rt_launch_shape{win.block, win.waves, win.refill_min, win.queue_kind};
-    return RT_OK;
-}
-
-rt_status rt_server_create(int device, const rt_server_config* cfg, rt_server** out) {
-    if (!out || !cfg) return fail(RT_ERR_INVALID_ARG, "null argument");
-    *out = nullptr;
-    if (cfg->hbm_budget_bytes < (1u << 20) || cfg->max_inflight == 0)
-        return fail(RT_ERR_INVALID_ARG, "server: hbm_budget_bytes >= 1 MiB and max_inflight > 0");
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(RT_ERR_NO_DEVICE, "no HIP device visible");
-    if (device < 0 || device >= n) return fail(RT_ERR_INVALID_ARG, "device index out of range");
-    rt_server* s = new (std::nothrow) rt_server();
-    if (!s) return fail(RT_ERR_OOM, "server allocation");
-    s->device = device;
-    s->cfg = *cfg;
-    RT_HIP(hipSetDevice(device));
-    s->pool_bytes = al256(cfg->hbm_budget_bytes);
-    hipError_t e = hipMalloc(&s->pool, s->pool_bytes);
-    if (e != hipSuccess) {
-        delete s;
-        return hip_fail(e, "hipMalloc(server pool)");
-    }
-    s->alloc.init(s->pool_bytes);
-    const int n_streams = 4;  // one per hardware queue
-    for (int i = 0; i < n_streams; ++i) {
-        hipStream_t st = nullptr;
-        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-        if (e != hipSuccess) break;
-        s->streams.push_back(st);
-        s->stream_busy.push_back(false);
-    }
-    if (e != hipSuccess) {
-        rt_server_destroy(s);
-        return hip_fail(e, "server streams");
-    }
-    s->worker = std::thread([s] { s->run(); });
-    *out = s;
-    return RT_OK;
-}
-
-rt_status rt_server_destroy(rt_server* s) {
-    if (!s) return RT_OK;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stop = true;
-    }
-    s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
-    (void)hipSetDevice(s->device);
-    for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
-    if (s->pool) (void)hipFree(s->pool);
-    delete s;
-    return RT_OK;
-}
-
-rt_status rt_server_submit(rt_server* s, const rt_request* req, uint64_t* id) {
-    if (!s || !req || !id) return fail(RT_ERR_INVALID_ARG, "null argument");
-    std::string why;
-    if (!valid_request(&req->metric, &req->integrator, &req->termination, why) || !valid_camera(&req->camera, why))
-        return fail(RT_ERR_INVALID_ARG, why);
-    TraceParams P;
-    fill_params(&req->metric, &req->integrator, &req->termination, P);
-    fill_camera(&req->camera, P, nullptr);
-    const size_t bytes = rt_server::footprint(P.n_rays, req->metric.n_sources >= 2);
-    if (bytes > s->pool_bytes)
-        return fail(RT_ERR_BUSY, "request footprint " + std::to_string(bytes) + " B exceeds the HBM budget");
-    std::lock_guard<std::mutex> lk(s->mu);
-    const uint64_t rid = s->next_id++;
-    ServerReq& r = s->reqs[rid];
-    r.req = *req;
-    r.n = P.n_rays;
-    r.bytes = bytes;
-    s->pending.push_back(rid);
-    *id = rid;
-    s->cv_work.notify_all();
-    return RT_OK;
-}
-
-rt_status rt_server_poll(rt_server* s, uint64_t id, int* done) {
-    if (!s || !done) return fail(RT_ERR_INVALID_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
-    const ReqState st = it->second.state;
-    *done = (st == ReqState::Done || st == ReqState::Cancelled || st == ReqState::Failed) ? 1 : 0;
-    return RT_OK;
-}
-
-rt_status rt_server_wait(rt_server* s, uint64_t id, int64_t timeout_ms) {
-    if (!s) return fail(RT_ERR_INVALID_ARG, "null server");
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto finished = [&]() {
-        auto it = s->reqs.find(id);
-        if (it == s->reqs.end()) return true;
-        const ReqState st = it->second.state;
-        return st == ReqState::Done || st == ReqState::Cancelled || st == ReqState::Failed;
-    };
-    if (timeout_ms < 0)
-        s->cv_done.wait(lk, finished);
-    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), finished))
-        return fail(RT_ERR_TIMEOUT, "request not finished before the deadline");
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
-    if (it->second.state == ReqState::Cancelled) return fail(RT_ERR_CANCELLED, "request cancelled");
-    if (it->second.state == ReqState::Failed) return fail(it->second.err, "request failed");
-    return RT_OK;
-}
-
-rt_status rt_server_take(rt_server* s, uint64_t id, rt_results* out) {
-    if (!s || !out) return fail(RT_ERR_INVALID_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
-    ServerReq& r = it->second;
-    if (r.state == ReqState::Cancelled) {
-        s->reqs.erase(it);
-        return fail(RT_ERR_CANCELLED, "request cancelled");
-    }
-    if (r.state == ReqState::Failed) {
-        const rt_status e = r.err;
-        s->reqs.erase(it);
-        return fail(e, "request failed");
-    }
-    if (r.state != ReqState::Done) return fail(RT_ERR_BUSY, "request not finished");
-    const size_t n = r.n;
-    if (out->cls) std::memcpy(out->cls, r.cls.data(), n * 4);
-    if (out->x) std::memcpy(out->x, r.x.data(), n * 32);
-    if (out->p) std::memcpy(out->p, r.p.data(), n * 32);
-    if (out->bl) std::memcpy(out->bl, r.bl.data(), n * 24);
-    if (out->lambda) std::memcpy(out->lambda, r.lambda.data(), n * 8);
-    if (out->g) std::memcpy(out->g, r.g.data(), n * 8);
-    if (out->steps) std::memcpy(out->steps, r.steps.data(), n * 4);
-    if (out->rejected) std::memcpy(out->rejected, r.rejected.data(), n * 4);
-    s->reqs.erase(it);
-    return RT_OK;
-}
-
-rt_status rt_server_cancel(rt_server* s, uint64_t id) {
-    if (!s) return fail(RT_ERR_INVALID_ARG, "null server");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return fail(RT_ERR_NOT_FOUND, "unknown request id");
-    it->second.cancel = true;
-    if (it->second.state == ReqState::Pending) {
-        s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), id), s->pending.end());
-        it->second.state = ReqState::Cancelled;
-        s->cv_done.notify_all();
-    }
-    s->cv_work.notify_all();
     return RT_OK;
 }
 

@@ -37,6 +37,16 @@ constexpr LaunchShape kMultiShape{64, 2, 64, 0};
 
 // Camera frame (single Kerr): one lane per pixel, lanes of a wave on an 8x8 tile.
 hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
+// The same frame as a persistent kernel of `grid` one-wave workgroups over its
+// own tile queue (*next zero at launch; grid is cut to the number of tiles).
+// limit/active as for launch_trace_multi.
+hipError_t launch_trace_camera_queue(const TraceParams& P, const OutPtrs& O, unsigned long long* next, uint32_t grid,
+                                     hipStream_t stream, const uint32_t* limit = nullptr,
+                                     unsigned long long* active = nullptr);
+// Wave slots of the device for the serving loop's kernels: one-wave
+// workgroups of the persistent camera and superposition kernels that can be
+// resident at once (compute units x workgroups per compute unit).
+hipError_t wave_slots(uint32_t* slots);
 // RT_METHOD_RENDER_RK4 camera frame, single Kerr (rt_render.hip).
 hipError_t launch_trace_render(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
 // Explicit initial states (O.in_x / O.in_p), single Kerr.
@@ -44,8 +54,12 @@ hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const Launc
 // Superposition (camera or explicit rays per P.mode), table T in device memory.
 // Lanes per ray of the superposition kernel (one source per lane).
 uint32_t multi_ray_lanes(uint32_t n_src);
+// max_grid > 0: at most that many workgroups (the serving loop's grant of
+// wave slots); 0: fill the device once. limit/active (serving loop): waves
+// leave when more than *limit of them run (*active counts them, zero at launch).
 hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
-                              const LaunchShape& L, hipStream_t stream);
+                              const LaunchShape& L, hipStream_t stream, uint32_t max_grid = 0,
+                              const uint32_t* limit = nullptr, unsigned long long* active = nullptr);
 // Persistent work-queue kernel over many single-Kerr requests (tiles or rays
 // per L.queue_kind 0 or 1; Q.req_offset counts tiles or rays accordingly).
 // grid 0: computed from the occupancy (not allowed inside a stream capture).
@@ -55,6 +69,24 @@ hipError_t launch_trace_queue(const QueueDev& Q, const LaunchShape& L, bool mixe
 hipError_t queue_grid(const LaunchShape& L, bool mixed, bool carter, uint32_t* grid);
 // Zero n 64-bit counters with a kernel (no memset node in captured graphs).
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
+// Byte ranges copied by one k_copy_segments launch; sources and destinations
+// are device memory or mapped pinned host memory, 4-byte aligned, sizes a
+// multiple of 4. The serving loop uploads a superposition's source table with
+// it, in the request's stream, instead of an asynchronous copy: the copy
+// engine's queue is shared by every stream, so a copy issued behind a long
+// kernel would hold back the copies of requests issued after it.
+constexpr int kMaxCopySegs = 8;
+struct CopySeg {
+    const void* src;
+    void* dst;
+    uint64_t bytes;
+};
+struct CopySegs {
+    CopySeg seg[kMaxCopySegs];
+    uint32_t n;
+};
+// At most max_blocks workgroups of 256 threads.
+hipError_t launch_copy_segments(const CopySegs& S, hipStream_t stream, uint32_t max_blocks);
 // Pack n results into 32-byte pixel records (rt_pixel layout) at out.
 hipError_t launch_pack_pixels(const OutPtrs& O, uint64_t n, void* out, hipStream_t stream);
 

@@ -562,6 +562,69 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// Wave-level yield of the serving loop's persistent kernels. *active counts
+// the request's running waves; a wave leaves when more are running than the
+// host's current limit for the request (a word in mapped pinned host memory,
+// read at system scope, lowered by the host to make room for other requests).
+// The host keeps the limit at 1 or more, so the request's last waves always
+// drain its queue. limit == nullptr: never yield.
+__device__ __forceinline__ bool wave_yields(const uint32_t* limit, unsigned long long* active) {
+    if (limit == nullptr) return false;
+    int go = 0;
+    if ((threadIdx.x & 63) == 0) {
+        const uint32_t lim = __hip_atomic_load(limit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const unsigned long long a = __hip_atomic_load(active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a > lim && atomicCAS(active, a, a - 1) == a) go = 1;
+    }
+    return __shfl(go, 0, 64) != 0;
+}
+
+__device__ __forceinline__ void wave_joins(const uint32_t* limit, unsigned long long* active) {
+    if (limit != nullptr && (threadIdx.x & 63) == 0) atomicAdd(active, 1ull);
+}
+
+// One camera request as a persistent kernel over its own tile queue: a wave
+// takes one 8x8 tile per atomicAdd on *next (zero at launch) and traces it as
+// k_trace_camera_kerr would, until the frame's tiles are gone. The grid is
+// whatever number of workgroups the caller grants (the serving loop's share
+// of the device's wave slots); every workgroup is resident at once, so the
+// kernel never holds the dispatcher while other requests' kernels wait. With
+// a limit, waves leave between tiles when the host lowers it (wave_yields).
+template <bool Mixed, bool Carter>
+__global__ __launch_bounds__(256, 2) void k_trace_camera_queue(TraceParams P, OutPtrs O,
+                                                               unsigned long long* __restrict__ next,
+                                                               uint32_t n_tiles, const uint32_t* limit,
+                                                               unsigned long long* active) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t rows = P.n_rays / P.cols;
+    Tally tally;
+    wave_joins(limit, active);
+    for (;;) {
+        if (wave_yields(limit, active)) break;
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(next, 1ull);
+        t = uniform_u64(__shfl(t, 0, 64));
+        if (t >= n_tiles) break;
+        uint32_t tx, ty;
+        tile_xy(P, static_cast<uint32_t>(t), tx, ty);
+        const uint32_t px = tx * 8 + (lane & 7);
+        const uint32_t py = ty * 8 + (lane >> 3);
+        if (px < P.cols && py < rows) {
+            double y0[8], xi;
+            camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
+            const KerrConst K = kerr_const(P.M, P.a, P.a2);
+            Lane L;
+            lane_begin<Carter>(P, K, y0, xi, L);
+            while (!lane_step<Mixed>(P, K, L)) {
+            }
+            lane_finish(L);
+            store_lane(O, static_cast<uint64_t>(py) * P.cols + px, L, P.a, P.a2);
+            tally.add(L);
+        }
+    }
+    tally.flush(O.counters);
+}
+
 // The request tables come in as separate read-only, non-aliased arguments so
 // that their wave-uniform loads stay scalar loads the compiler may repeat
 // instead of holding the parameters in vector registers.
@@ -1143,7 +1206,8 @@ template <int G, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPtrs O,
                                                             const MultiDev* __restrict__ Tg,
                                                             unsigned long long* __restrict__ next,
-                                                            uint32_t refill_min) {
+                                                            uint32_t refill_min, const uint32_t* limit,
+                                                            unsigned long long* active_waves) {
     __shared__ MultiDev sm;
     {
         const uint32_t words = sizeof(MultiDev) / 8;
@@ -1164,9 +1228,14 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
     bool active = false, drained = false;
     uint64_t idx = 0;
     Tally tally;
+    wave_joins(limit, active_waves);
     for (;;) {
         const uint64_t idle = __ballot(!active && si == 0);
         const uint32_t n_idle = __popcll(idle);
+        // the host asks for fewer waves: take no more rays, leave once idle
+        if (!drained && n_idle != 0 && (n_idle * G >= refill_min || n_idle * G == 64) &&
+            wave_yields(limit, active_waves))
+            drained = true;
         if (!drained && n_idle != 0 && (n_idle * G >= refill_min || n_idle * G == 64)) {
             const uint32_t first = __ffsll(static_cast<unsigned long long>(idle)) - 1;
             unsigned long long base = 0;
@@ -1252,6 +1321,46 @@ hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const Lau
     return P.mixed ? launch_camera_t<true, false>(P, O, L, stream) : launch_camera_t<false, false>(P, O, L, stream);
 }
 
+hipError_t launch_trace_camera_queue(const TraceParams& P, const OutPtrs& O, unsigned long long* next,
+                                     uint32_t grid, hipStream_t stream, const uint32_t* limit,
+                                     unsigned long long* active) {
+    if (P.cols == 0 || P.n_rays % P.cols != 0 || grid == 0) return hipErrorInvalidValue;
+    const uint64_t tiles = static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.cols + 7) / 8);
+    if (tiles > 0xffffffffull) return hipErrorInvalidValue;
+    const uint32_t n_tiles = static_cast<uint32_t>(tiles);
+    grid = std::min(grid, n_tiles);
+    if (P.carter) {
+        if (P.mixed)
+            hipLaunchKernelGGL((k_trace_camera_queue<true, true>), dim3(grid), dim3(64), 0, stream, P, O, next, n_tiles,
+                               limit, active);
+        else
+            hipLaunchKernelGGL((k_trace_camera_queue<false, true>), dim3(grid), dim3(64), 0, stream, P, O, next, n_tiles,
+                               limit, active);
+    } else if (P.mixed) {
+        hipLaunchKernelGGL((k_trace_camera_queue<true, false>), dim3(grid), dim3(64), 0, stream, P, O, next, n_tiles,
+                               limit, active);
+    } else {
+        hipLaunchKernelGGL((k_trace_camera_queue<false, false>), dim3(grid), dim3(64), 0, stream, P, O, next, n_tiles,
+                               limit, active);
+    }
+    return hipGetLastError();
+}
+
+hipError_t wave_slots(uint32_t* slots) {
+    int dev = 0, cus = 0, a = 0, b = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &a, reinterpret_cast<const void*>(&k_trace_camera_queue<false, false>), 64, 0);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(&k_trace_multi<16, 2>),
+                                                         64, 0);
+    if (e != hipSuccess) return e;
+    *slots = static_cast<uint32_t>(cus) * static_cast<uint32_t>(std::max(1, std::min(a, b)));
+    return hipSuccess;
+}
+
 template <bool Mixed, bool Carter>
 static hipError_t launch_rays_t(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream) {
     const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(P.n_rays) + L.block - 1) / L.block);
@@ -1279,7 +1388,8 @@ uint32_t multi_ray_lanes(uint32_t n_src) {
 
 template <int G>
 static hipError_t launch_multi_g(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
-                                 const LaunchShape& L, hipStream_t stream) {
+                                 const LaunchShape& L, hipStream_t stream, uint32_t max_grid, const uint32_t* limit,
+                                 unsigned long long* active) {
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1291,26 +1401,30 @@ static hipError_t launch_multi_g(const TraceParams& P, const OutPtrs& O, const M
     const uint64_t rays = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.cols + 7) / 8) * 64
                                         : static_cast<uint64_t>(P.n_rays);
     const uint64_t want = (rays * G + L.block - 1) / L.block;
-    const uint32_t grid = static_cast<uint32_t>(
-        std::min<uint64_t>(want, static_cast<uint64_t>(cus) * static_cast<uint64_t>(per_cu > 0 ? per_cu : 1)));
+    uint64_t grid64 = std::min<uint64_t>(want, static_cast<uint64_t>(cus) * static_cast<uint64_t>(per_cu > 0 ? per_cu : 1));
+    if (max_grid != 0) grid64 = std::min<uint64_t>(grid64, max_grid);
+    const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(grid64, 1));
     const uint32_t refill = std::max<uint32_t>(1, std::min<uint32_t>(L.refill_min, 64));
     if (L.waves >= 2)
-        hipLaunchKernelGGL((k_trace_multi<G, 2>), dim3(grid), dim3(L.block), 0, stream, P, O, T, next, refill);
+        hipLaunchKernelGGL((k_trace_multi<G, 2>), dim3(grid), dim3(L.block), 0, stream, P, O, T, next, refill, limit,
+                           active);
     else
-        hipLaunchKernelGGL((k_trace_multi<G, 1>), dim3(grid), dim3(L.block), 0, stream, P, O, T, next, refill);
+        hipLaunchKernelGGL((k_trace_multi<G, 1>), dim3(grid), dim3(L.block), 0, stream, P, O, T, next, refill, limit,
+                           active);
     return hipGetLastError();
 }
 
 hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const MultiDev* T, unsigned long long* next,
-                              const LaunchShape& L, hipStream_t stream) {
+                              const LaunchShape& L, hipStream_t stream, uint32_t max_grid, const uint32_t* limit,
+                              unsigned long long* active) {
     if (!shape_ok(L)) return hipErrorInvalidValue;
     if (P.mode == 0 && (P.cols == 0 || P.n_rays % P.cols != 0)) return hipErrorInvalidValue;
     if (P.n_src < 2 || P.n_src > static_cast<uint32_t>(kMaxSources)) return hipErrorInvalidValue;
     switch (multi_ray_lanes(P.n_src)) {
-        case 2: return launch_multi_g<2>(P, O, T, next, L, stream);
-        case 4: return launch_multi_g<4>(P, O, T, next, L, stream);
-        case 8: return launch_multi_g<8>(P, O, T, next, L, stream);
-        case 16: return launch_multi_g<16>(P, O, T, next, L, stream);
+        case 2: return launch_multi_g<2>(P, O, T, next, L, stream, max_grid, limit, active);
+        case 4: return launch_multi_g<4>(P, O, T, next, L, stream, max_grid, limit, active);
+        case 8: return launch_multi_g<8>(P, O, T, next, L, stream, max_grid, limit, active);
+        case 16: return launch_multi_g<16>(P, O, T, next, L, stream, max_grid, limit, active);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1394,6 +1508,48 @@ hipError_t launch_pack_pixels(const OutPtrs& O, uint64_t n, void* out, hipStream
 
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream) {
     hipLaunchKernelGGL(k_zero_u64, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+// Copy up to kMaxCopySegs byte ranges (device <-> mapped pinned host) in one
+// launch: 16-byte moves where source and destination are both 16-byte
+// aligned, the rest in 4-byte moves. Grid-stride over each segment in turn.
+__global__ void k_copy_segments(CopySegs S) {
+    const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint32_t k = 0; k < S.n; ++k) {
+        const char* src = static_cast<const char*>(S.seg[k].src);
+        char* dst = static_cast<char*>(S.seg[k].dst);
+        const uint64_t bytes = S.seg[k].bytes;
+        uint64_t vec = 0;
+        if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0) {
+            vec = bytes / 16;
+            const uint4* s4 = reinterpret_cast<const uint4*>(src);
+            uint4* d4 = reinterpret_cast<uint4*>(dst);
+            for (uint64_t i = tid; i < vec; i += stride) d4[i] = s4[i];
+        }
+        const uint64_t done = vec * 16;
+        const uint64_t words = (bytes - done) / 4;
+        const uint32_t* s1 = reinterpret_cast<const uint32_t*>(src + done);
+        uint32_t* d1 = reinterpret_cast<uint32_t*>(dst + done);
+        for (uint64_t i = tid; i < words; i += stride) d1[i] = s1[i];
+    }
+}
+
+hipError_t launch_copy_segments(const CopySegs& S, hipStream_t stream, uint32_t max_blocks) {
+    if (S.n > static_cast<uint32_t>(kMaxCopySegs)) return hipErrorInvalidValue;
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < S.n; ++k) {
+        if ((S.seg[k].bytes & 3) != 0 || (reinterpret_cast<uintptr_t>(S.seg[k].src) & 3) != 0 ||
+            (reinterpret_cast<uintptr_t>(S.seg[k].dst) & 3) != 0)
+            return hipErrorInvalidValue;
+        total += S.seg[k].bytes;
+    }
+    if (total == 0) return hipSuccess;
+    const uint64_t want = (total / 16 + 255) / 256;
+    const uint32_t grid = static_cast<uint32_t>(
+        std::max<uint64_t>(1, std::min<uint64_t>(want, std::max<uint32_t>(1, std::min<uint32_t>(max_blocks, 256)))));
+    hipLaunchKernelGGL(k_copy_segments, dim3(grid), dim3(256), 0, stream, S);
     return hipGetLastError();
 }
 

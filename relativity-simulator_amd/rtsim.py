@@ -53,9 +53,23 @@ RT_PROJ_FISHEYE_ORTHOGRAPHIC = 5
 RT_PROJ_PANINI_CYLINDRICAL = 6
 RT_PROJ_HAMMER_AITOFF = 7
 
+RT_OUT_CLS = 1 << 0
+RT_OUT_X = 1 << 1
+RT_OUT_P = 1 << 2
+RT_OUT_BL = 1 << 3
+RT_OUT_LAMBDA = 1 << 4
+RT_OUT_G = 1 << 5
+RT_OUT_STEPS = 1 << 6
+RT_OUT_REJECTED = 1 << 7
+RT_OUT_ALL = 0xFF
+RT_OUT_PIXEL = RT_OUT_CLS | RT_OUT_LAMBDA | RT_OUT_G | RT_OUT_STEPS
+# result fields in rt_results order, with their RT_OUT_* bit
+OUT_FIELDS = (("cls", RT_OUT_CLS), ("x", RT_OUT_X), ("p", RT_OUT_P), ("bl", RT_OUT_BL), ("lam", RT_OUT_LAMBDA),
+              ("g", RT_OUT_G), ("steps", RT_OUT_STEPS), ("rejected", RT_OUT_REJECTED))
+
 STATUS_NAMES = {
     0: "RT_OK", 1: "RT_ERR_INVALID_ARG", 2: "RT_ERR_OOM", 3: "RT_ERR_DEVICE", 4: "RT_ERR_TIMEOUT",
-    5: "RT_ERR_CANCELLED", 6: "RT_ERR_NO_DEVICE", 7: "RT_ERR_BUSY", 8: "RT_ERR_NOT_FOUND",
+    5: "RT_ERR_CANCELLED", 6: "RT_ERR_NO_DEVICE", 7: "RT_ERR_BUSY", 8: "RT_ERR_NOT_FOUND", 9: "RT_ERR_EXPIRED",
 }
 
 
@@ -167,7 +181,11 @@ class rt_launch_shape(C.Structure):
 
 
 class rt_server_config(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32), ("slots", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32), ("staging_mib", C.c_uint32)]
+
+
+class rt_submit_options(C.Structure):
+    _fields_ = [("outputs", C.c_uint32), ("reserved", C.c_uint32), ("deadline_ms", C.c_int64)]
 
 
 # ------------------------------------------------------------ defaults
@@ -305,7 +323,7 @@ EXPORTS = [
     "rt_abi_version", "rt_version", "rt_last_error", "rt_context_create", "rt_context_destroy",
     "rt_get_stats", "rt_trace_camera", "rt_trace_rays", "rt_trace_camera_resident", "rt_fetch_results",
     "rt_fetch_pixels", "rt_tile_geometry", "rt_decode_tile_task", "rt_trace_tile", "rt_trace_tiles",
-    "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_poll",
+    "rt_trace_batch", "rt_get_launch_shape", "rt_set_launch_shape", "rt_autotune", "rt_server_create", "rt_server_destroy", "rt_server_submit", "rt_server_submit_ex", "rt_server_poll",
     "rt_server_wait", "rt_server_take", "rt_server_cancel", "rt_trace_tiles_noalloc", "rt_critical_distance",
     "rt_shadow_boundary", "rt_impact_distance",
 ]
@@ -358,6 +376,7 @@ def load(path: Optional[str] = None) -> C.CDLL:
     lib.rt_server_create.argtypes = [C.c_int, P(rt_server_config), P(C.c_void_p)]
     lib.rt_server_destroy.argtypes = [C.c_void_p]
     lib.rt_server_submit.argtypes = [C.c_void_p, P(rt_request), P(C.c_uint64)]
+    lib.rt_server_submit_ex.argtypes = [C.c_void_p, P(rt_request), P(rt_submit_options), P(C.c_uint64)]
     lib.rt_server_poll.argtypes = [C.c_void_p, C.c_uint64, P(C.c_int)]
     lib.rt_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_int64]
     lib.rt_server_take.argtypes = [C.c_void_p, C.c_uint64, P(rt_results)]
@@ -526,15 +545,16 @@ class Context:
 
 
 class Server:
-    """Continuous-batching request loop on one device."""
+    """Request loop on one device (rt_server_*): requests run as soon as memory
+    and a lane allow, next to the ones already on the device."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, max_inflight: int = 64,
-                 slots: int = 0):
+                 staging_mib: int = 0):
         self.lib = load()
-        cfg = rt_server_config(hbm_budget_bytes, max_inflight, slots)
+        cfg = rt_server_config(hbm_budget_bytes, max_inflight, staging_mib)
         self.h = C.c_void_p()
         _check(self.lib, self.lib.rt_server_create(device, C.byref(cfg), C.byref(self.h)))
-        self._sizes: Dict[int, int] = {}
+        self._sizes: Dict[int, tuple] = {}
 
     def close(self) -> None:
         if self.h:
@@ -547,10 +567,15 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
-    def submit(self, req: rt_request) -> int:
+    def submit(self, req: rt_request, outputs: int = RT_OUT_ALL, deadline_ms: int = 0) -> int:
+        """Queue a request; outputs: RT_OUT_* bits to copy back; deadline_ms > 0: expire after it."""
         rid = C.c_uint64()
-        _check(self.lib, self.lib.rt_server_submit(self.h, C.byref(req), C.byref(rid)))
-        self._sizes[rid.value] = n_rays_of(req.camera)
+        if outputs == RT_OUT_ALL and deadline_ms == 0:
+            _check(self.lib, self.lib.rt_server_submit(self.h, C.byref(req), C.byref(rid)))
+        else:
+            opt = rt_submit_options(outputs, 0, deadline_ms)
+            _check(self.lib, self.lib.rt_server_submit_ex(self.h, C.byref(req), C.byref(opt), C.byref(rid)))
+        self._sizes[rid.value] = (n_rays_of(req.camera), outputs)
         return rid.value
 
     def poll(self, rid: int) -> bool:
@@ -562,8 +587,16 @@ class Server:
         _check(self.lib, self.lib.rt_server_wait(self.h, rid, timeout_ms))
 
     def take(self, rid: int) -> Results:
-        res = Results(self._sizes.pop(rid))
-        _check(self.lib, self.lib.rt_server_take(self.h, rid, C.byref(res.struct)))
+        """Results of a finished request; fields not requested at submit stay zero."""
+        n, outputs = self._sizes.get(rid, (0, RT_OUT_ALL))
+        res = Results(n)
+        for name, bit in OUT_FIELDS:
+            if not outputs & bit:
+                setattr(res.struct, name, None)
+        st = self.lib.rt_server_take(self.h, rid, C.byref(res.struct))
+        if st != 7:  # RT_ERR_BUSY: still running, keep the bookkeeping
+            self._sizes.pop(rid, None)
+        _check(self.lib, st)
         return res
 
     def cancel(self, rid: int) -> None:

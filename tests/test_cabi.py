@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported():
 
 STRUCTS = ["rt_source", "rt_metric", "rt_integrator", "rt_termination", "rt_camera", "rt_results", "rt_stats",
            "rt_request", "rt_server_config", "rt_launch_shape", "rt_pixel", "rt_tile", "rt_tile_task",
-           "rt_disk_profile"]
+           "rt_disk_profile", "rt_submit_options"]
 
 
 def test_struct_layout_matches_ctypes():
@@ -48,7 +48,8 @@ def test_struct_layout_matches_ctypes():
                  ("rt_camera", "projection"), ("rt_integrator", "method"), ("rt_integrator", "far_field_step_scale"),
                  ("rt_termination", "time"), ("rt_termination", "disk_profile"), ("rt_disk_profile", "edge_softness"),
                  ("rt_pixel", "r"), ("rt_pixel", "g"), ("rt_pixel", "b"), ("rt_pixel", "a"),
-                 ("rt_pixel", "affine_parameter"), ("rt_pixel", "status_flags")]:
+                 ("rt_pixel", "affine_parameter"), ("rt_pixel", "status_flags"),
+                 ("rt_server_config", "staging_mib"), ("rt_submit_options", "deadline_ms")]:
         src += f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));\n'
     src += "return 0;}\n"
     with tempfile.TemporaryDirectory() as d:
@@ -66,6 +67,8 @@ def test_struct_layout_matches_ctypes():
     assert int(got["rt_request.termination"]) == rtsim.rt_request.termination.offset
     assert int(got["rt_stats.block_size"]) == rtsim.rt_stats.block_size.offset
     assert int(got["rt_results.rejected"]) == rtsim.rt_results.rejected.offset
+    assert int(got["rt_server_config.staging_mib"]) == rtsim.rt_server_config.staging_mib.offset
+    assert int(got["rt_submit_options.deadline_ms"]) == rtsim.rt_submit_options.deadline_ms.offset
     # rt_pixel against the reference's GpuPixelOutput (render/gpu_types.hpp:1111-1121): the offsets and
     # size that offsetof/sizeof give on the reference's struct (tests/golden/render_frames.json)
     ref_layout = json.load(open(os.path.join(REPO, "tests", "golden", "render_frames.json")))["pixel_layout"]
@@ -100,7 +103,7 @@ def test_defaults_match_python_mirror():
     pc = rtsim.default_camera(1000.0, 1.0, 0.3, 0.35, 64, 32)
     for f in ("observer", "forward", "right", "up"):
         assert np.allclose(list(getattr(cam, f)), list(getattr(pc, f)), rtol=0, atol=1e-15)
-    assert lib.rt_abi_version() == 2
+    assert lib.rt_abi_version() == 3
     assert b"librtsim" in lib.rt_version()
 
 

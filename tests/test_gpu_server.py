@@ -1,9 +1,13 @@
-"""Serving loop: continuous batching on the device, admission by HBM footprint.
+"""Serving loop (rt_server_*): requests admitted by HBM footprint run at once
+next to those on the device.
 
 Every request's result must be the same alone (single-request kernel) and in
 the mix of concurrently served requests (class identical, coordinates within
 1e-12 relative); a request too large for the budget is refused with
-RT_ERR_BUSY; a cancelled pending request reports RT_ERR_CANCELLED."""
+RT_ERR_BUSY; a cancelled request reports RT_ERR_CANCELLED and one past its
+deadline RT_ERR_EXPIRED; only the outputs asked for come back. The request
+states, admission and deadlines themselves are checked on the CPU with an
+injected clock (tests/test_serve_core.py)."""
 import math
 import time
 
@@ -55,3 +59,51 @@ def test_admission_and_cancel():
         assert e2.value.status == 5
         with pytest.raises(rtsim.RtError):
             srv.poll(123456)
+
+
+def test_outputs_subset_and_take_errors(ctx):
+    """Only the outputs asked for at submit come back (the pixel record's four
+    here), equal to the full run's; asking take for another output is refused
+    and leaves the request in place."""
+    import ctypes as C
+    r = req(64, 0.9, 70)
+    with rtsim.Server(0, hbm_budget_bytes=64 << 20, max_inflight=8) as srv:
+        full_id = srv.submit(r)
+        pix_id = srv.submit(r, outputs=rtsim.RT_OUT_PIXEL)
+        srv.wait(full_id, 60000)
+        srv.wait(pix_id, 60000)
+        full = srv.take(full_id)
+        n = rtsim.n_rays_of(r.camera)
+        bad = rtsim.Results(n)  # every field requested: x, p, bl and rejected were not asked for
+        assert srv.lib.rt_server_take(srv.h, pix_id, C.byref(bad.struct)) == 1
+        pix = srv.take(pix_id)
+    for f in ("cls", "lam", "g", "steps"):
+        assert np.array_equal(getattr(pix, f), getattr(full, f)), f
+    assert not pix.x.any() and not pix.p.any() and not pix.rejected.any()
+
+
+def test_deadline_cancel_and_shutdown_with_work_on_the_device(ctx):
+    """A deadline shorter than the trace ends the request RT_ERR_EXPIRED; a
+    cancelled large request ends RT_ERR_CANCELLED; requests queued after them
+    still finish with the right results; destroying the server with work on
+    the device waits for it."""
+    big = req(1024, 0.998, 85)
+    small = req(48, 0.5, 60)
+    alone = ctx.trace_camera(small.camera, small.metric, small.integrator, small.termination)
+    with rtsim.Server(0, hbm_budget_bytes=1 << 30, max_inflight=8) as srv:
+        expired = srv.submit(big, deadline_ms=1)
+        cancelled = srv.submit(big)
+        srv.cancel(cancelled)
+        after = srv.submit(small)
+        for rid, status in ((expired, 9), (cancelled, 5)):
+            with pytest.raises(rtsim.RtError) as e:
+                srv.wait(rid, 60000)
+            assert e.value.status == status
+            with pytest.raises(rtsim.RtError) as e2:
+                srv.take(rid)
+            assert e2.value.status == status
+        srv.wait(after, 60000)
+        got = srv.take(after)
+        assert np.array_equal(got.cls, alone.cls) and np.array_equal(got.x, alone.x)
+        for _ in range(3):
+            srv.submit(big)  # left on the device: the server's exit waits for them
