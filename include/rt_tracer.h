@@ -414,9 +414,10 @@ rt_status rt_trace_tile(rt_context* ctx, const rt_camera* cam, const rt_metric* 
  * captured in a hipGraph that is replayed while the batch layout stays the
  * same. Results stay device-resident, task after task, each tile-local
  * row-major, for rt_fetch_results / rt_fetch_pixels; *n_rays receives their
- * total. Each task's results are rt_trace_tile's for it to rounding (class,
- * flags and step counts identical; another kernel, the same per-ray
- * arithmetic).
+ * total. Each task's results equal rt_trace_tile's to 1e-12 relative, with
+ * class, flags and step counts identical: the same formulas in another
+ * kernel, which the compiler contracts into FMAs differently, so the last
+ * bits can differ.
  * Replaces the per-task loop of a tile worker, TileRenderer::render called
  * once per RenderTileTaskPayload (src/distributed/tile_renderer.cpp:133-182),
  * whose GPU dispatches gpu_mutex_ serialises. */
