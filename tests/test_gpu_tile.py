@@ -130,8 +130,9 @@ def test_tile_batch_equals_single_tiles(ctx, batch_kind):
     """A batch of tasks of different frames, metrics (one a superposition),
     tolerances and tile shapes: each task's slice of the batch equals
     rt_trace_tile of it (class, flags and step counts identical, coordinates
-    within 1e-12: the shared tile-queue kernel runs the same per-ray
-    arithmetic), on two replays and after a smaller batch."""
+    within 1e-12: the queue kernels run the same formulas, which the compiler
+    contracts into FMAs differently), on two replays and after a smaller
+    batch."""
     names = ["kerr_centre_tile", "ragged_corner_tile", "worker_test_task", "scaled_disk_tile", "disk_off_tile"]
     batch = [fixture_task(n) for n in names]
     multi = fixture_task("kerr_centre_tile")
