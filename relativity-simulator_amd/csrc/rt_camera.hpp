@@ -115,16 +115,36 @@ __host__ __device__ __forceinline__ void screen_direction(const TraceParams& P, 
     }
 }
 
+// Every projection other than the pinhole one, out of line on the device.
+// The persistent kernels map pixels inside their ray loop; inlined there, the
+// library constants of these projections (sin, cos, atan, asin polynomials)
+// would be hoisted out of the loop and held in registers for the kernel's
+// whole life.
+static __device__ __noinline__ void screen_direction_out_of_line(const TraceParams& P, double u, double v,
+                                                                 double nl[3]) {
+    screen_direction(P, u, v, nl);
+}
+
 // Camera pixel (px, py) -> the reference's `outward` vector: screen point,
 // ray direction, world direction from the camera basis, projection onto
 // e_r / e_theta / e_phi at the observer and back to Cartesian.
+// InLoop (device code inside a ray loop): the pinhole projection inline, the
+// others through screen_direction_out_of_line; same results either way.
+template <bool InLoop = false>
 __host__ __device__ __forceinline__ void camera_outward(const TraceParams& P, uint32_t px, uint32_t py, double o[3]) {
     const double v = 1.0 - (static_cast<double>(py) + 0.5) / static_cast<double>(P.height) * 2.0;
     const double xs = (static_cast<double>(px) + 0.5) / static_cast<double>(P.width) * 2.0 - 1.0;
     const bool allsky = P.projection == RT_PROJ_EQUIRECTANGULAR_360 || P.projection == RT_PROJ_HAMMER_AITOFF;
     const double u = allsky ? xs : xs * P.aspect;
     double l[3];
-    screen_direction(P, u, v, l);
+    if constexpr (InLoop) {
+        if (P.projection == RT_PROJ_PINHOLE || P.projection == RT_PROJ_AUTO_ZOOM)
+            unit3(1.0, v * P.tan_half, u * P.tan_half, l);
+        else
+            screen_direction_out_of_line(P, u, v, l);
+    } else {
+        screen_direction(P, u, v, l);
+    }
     const double d0 = l[0] * P.fwd[0] + l[2] * P.right[0] + l[1] * P.up[0];
     const double d1 = l[0] * P.fwd[1] + l[2] * P.right[1] + l[1] * P.up[1];
     const double d2 = l[0] * P.fwd[2] + l[2] * P.right[2] + l[1] * P.up[2];
@@ -140,10 +160,11 @@ __host__ __device__ __forceinline__ void camera_outward(const TraceParams& P, ui
 // Photon momentum -outward in the observer's Gram–Schmidt frame, normalised
 // to p_t = -1 (kerr_schild_null_integrator.hpp:179-193). The frame is the same
 // for every pixel and comes precomputed in P.
+template <bool InLoop = false>
 __host__ __device__ __forceinline__ void camera_state_kerr(const TraceParams& P, uint32_t px, uint32_t py, double y[8],
-                                                  double* xi) {
+                                                           double* xi) {
     double o[3];
-    camera_outward(P, px, py, o);
+    camera_outward<InLoop>(P, px, py, o);
     double vv[4] = {P.u0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {

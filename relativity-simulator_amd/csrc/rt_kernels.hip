@@ -13,8 +13,9 @@
 //                        a wave refills its idle lanes with one atomic
 //                        (ballot + prefix count)
 //   k_trace_multi        superposition of up to 16 sources: G lanes per ray
-//                        (one source each, sums by DPP butterflies),
-//                        persistent, idle groups refilled ray by ray
+//                        (one source and a share of the state components
+//                        each, sums by DPP butterflies), persistent, idle
+//                        groups refilled ray by ray
 #include <hip/hip_runtime.h>
 
 #include "rt_camera.hpp"
@@ -385,7 +386,8 @@ __device__ __forceinline__ void store_lane(const OutPtrs& O, uint64_t idx, const
 
 struct Tally {
     unsigned long long acc = 0, rej = 0, rhs = 0, f32 = 0, proj = 0;
-    __device__ void add(const Lane& L) {
+    template <class LaneT>
+    __device__ void add(const LaneT& L) {
         acc += L.acc;
         rej += L.rej;
         rhs += L.rhs;
@@ -515,7 +517,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_queue(QueueDev Q) {
                         const uint32_t py = ty * 8 + (l >> 3);
                         ok = (px < P.cols) && (py < P.n_rays / P.cols);
                         if (ok) {
-                            camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
+                            camera_state_kerr<true>(P, P.col_first + px, P.row_first + py, y0, &xi);
                             ray = static_cast<uint64_t>(py) * P.cols + px;
                         }
                     } else {
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(256, 2) void k_trace_camera_queue(TraceParams P, Ou
         const uint32_t py = ty * 8 + (lane >> 3);
         if (px < P.cols && py < rows) {
             double y0[8], xi;
-            camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
+            camera_state_kerr<true>(P, P.col_first + px, P.row_first + py, y0, &xi);
             const KerrConst K = kerr_const(P.M, P.a, P.a2);
             Lane L;
             lane_begin<Carter>(P, K, y0, xi, L);
@@ -652,7 +654,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
         uint32_t c_acc = 0, c_rej = 0, c_rhs = 0, c_f32 = 0, c_proj = 0;
         if (px < P.cols && py < P.n_rays / P.cols) {
             double y0[8], xi = 0.0;
-            camera_state_kerr(P, P.col_first + px, P.row_first + py, y0, &xi);
+            camera_state_kerr<true>(P, P.col_first + px, P.row_first + py, y0, &xi);
             const KerrConst K = kerr_const(P.M, P.a, P.a2);
             Lane L;
             lane_begin<Carter>(P, K, y0, xi, L);
@@ -682,25 +684,35 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
 }
 
 // ------------------------------------------------------- superposition
-// Source-parallel layout. The G lanes of a group (G = 2, 4, 8 or 16, the
-// smallest power of two not below the number of sources) trace one ray
-// together: every lane holds the whole ray state, lane s of the group
-// evaluates source s, and sums over the sources are butterfly reductions
-// inside the group (DPP moves, no memory). Floating-point addition is
-// commutative bit for bit, so after each butterfly level both partners hold
-// identical bits: every lane of a group ends with the same sum and the lanes
-// take the same accept/reject and event decisions in lock step. A ray's
-// latency per step is then that of one source instead of all of them, which
-// matters because a superposition frame's time is set by its slowest rays
-// (rays that graze a horizon take thousands of steps).
+// Component-distributed layout. The G lanes of a group (G = 2, 4, 8 or 16,
+// the smallest power of two not below the number of sources) trace one ray
+// together. Lane l of the group evaluates source l, and owns the state
+// components c = q + Gc m (m < C) of the DP45 step, with q = l mod Gc,
+// Gc = min(G, 8) and C = 8 / Gc: it keeps their values, their seven stages
+// and their stage sums, so no lane holds the whole step (for G = 16 the two
+// halves of the group own the same components). Per right-hand side the
+// group gathers the position and momentum from their owners (every lane
+// needs them for its source), each lane adds its source's terms, and the
+// sums over the sources are reduced onto the owners of the components
+// (reduce-scatter by DPP butterflies, no memory). Values every lane needs
+// (p_t, the current position and momentum, the step size, counters) are kept
+// in every lane. Floating-point addition is commutative bit for bit, so both
+// partners of a butterfly level hold identical bits: every group-wide value
+// is the same in all lanes of the group, and the lanes take the same
+// accept/reject and event decisions in lock step. A ray's latency per step is
+// that of one source and one share of the stage arithmetic, which matters
+// because a superposition frame's time is set by its slowest rays (rays that
+// graze a horizon take thousands of steps).
 
 // DPP lane moves inside rows of 16: xor 1 and xor 2 (quad permutes), mirror
-// inside 8 lanes and inside 16 lanes. Each pairs every lane with a partner
-// in the same aligned group of 2, 4, 8 or 16 lanes.
+// inside 8 lanes and inside 16 lanes, rotation by 8 inside 16 lanes (= xor 8).
+// Each pairs every lane with a partner in the same aligned group of 2, 4, 8
+// or 16 lanes.
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141; // row_half_mirror
 constexpr int kDppMirror = 0x140;     // row_mirror
+constexpr int kDppRor8 = 0x128;       // row_ror:8
 
 template <int Ctrl>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -715,6 +727,15 @@ __device__ __forceinline__ double dpp_f64(double v) {
 template <int Ctrl>
 __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), Ctrl, 0xF, 0xF, false));
+}
+
+// The value of v in the lane whose byte address (4 x lane) is addr.
+__device__ __forceinline__ double bperm_f64(int addr, double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, static_cast<int>(static_cast<uint32_t>(b)));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, static_cast<int>(static_cast<uint32_t>(b >> 32)));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) |
+                                          static_cast<uint32_t>(lo));
 }
 
 // Sum over the group (identical bits in every lane of the group).
@@ -759,6 +780,135 @@ __device__ __forceinline__ void group_argmin(double& s, uint32_t& i) {
     if constexpr (G >= 16) argmin_step<kDppMirror>(s, i);
 }
 
+// Layout constants of a group of G lanes.
+template <int G>
+struct MG {
+    static constexpr int Gc = G < 8 ? G : 8;  // lanes owning distinct components
+    static constexpr int C = 8 / Gc;          // components per lane
+};
+
+// Per-lane constants of the group layout.
+struct MCtx {
+    uint32_t q;   // component slot: the lane owns components q + Gc m
+    int bp;       // 4 x (first lane of the group): ds_bpermute address of the group's first lane
+};
+
+// Component c (compile time) of a group-distributed vector v, in every lane of
+// the group: read from its owner (lane q_c = c mod Gc of the group, register
+// m = c / Gc). Pairs and quads broadcast by a quad permute; groups of 8 and 16
+// read the lower half's owner through the LDS crossbar.
+template <int G, int c>
+__device__ __forceinline__ double from_owner(const double* v, const MCtx& cx) {
+    constexpr int Gc = MG<G>::Gc;
+    constexpr int qc = c % Gc, m = c / Gc;
+    if constexpr (Gc == 2)
+        return dpp_f64<(qc == 0) ? 0xA0 : 0xF5>(v[m]);  // quad_perm [0,0,2,2] / [1,1,3,3]
+    else if constexpr (Gc == 4)
+        return dpp_f64<qc * 0x55>(v[m]);  // quad_perm [qc,qc,qc,qc]
+    else
+        return bperm_f64(cx.bp + 4 * qc, v[m]);
+}
+
+// Sum over the group of the 8-slot vectors v (slot = component), delivered to
+// the owners: S[m] = sum over the group's lanes of v[q + Gc m]. Each level
+// halves the slots a lane keeps: it sends its partner the half the partner
+// keeps and adds the half it receives to its own.
+template <int G>
+__device__ __forceinline__ void reduce_scatter(double v[8], uint32_t q, double* S) {
+    constexpr int Gc = MG<G>::Gc;
+    if constexpr (G == 16) {
+        // the two halves of the row (lanes q and q + 8) then hold the same partial sums
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c != 4) v[c] = v[c] + dpp_f64<kDppRor8>(v[c]);
+    }
+    if constexpr (Gc == 8) {
+        // lanes 0-3 keep components 0-3, lanes 4-7 components 4-7 (partner 7 - q)
+        const bool lo = q < 4;
+        double w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double own = lo ? v[k] : v[4 + k];
+            const double send = lo ? v[4 + k] : v[k];
+            w[k] = own + dpp_f64<kDppHalfMirror>(send);
+        }
+        // partner q ^ 2: keep w[0..1] or w[2..3]
+        const bool b1 = (q & 2) == 0;
+        double u[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const double own = b1 ? w[k] : w[2 + k];
+            const double send = b1 ? w[2 + k] : w[k];
+            u[k] = own + dpp_f64<kDppXor2>(send);
+        }
+        // partner q ^ 1: component q
+        const bool b0 = (q & 1) == 0;
+        S[0] = (b0 ? u[0] : u[1]) + dpp_f64<kDppXor1>(b0 ? u[1] : u[0]);
+    } else if constexpr (Gc == 4) {
+        // partner q ^ 2: lanes 0, 1 keep components {0, 1, 4, 5}, lanes 2, 3 {2, 3, 6, 7}
+        const bool b1 = (q & 2) == 0;
+        double w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int ca = (k & 1) + 4 * (k >> 1);
+            const double own = b1 ? v[ca] : v[ca + 2];
+            const double send = b1 ? v[ca + 2] : v[ca];
+            w[k] = own + dpp_f64<kDppXor2>(send);
+        }
+        // partner q ^ 1: components q and q + 4
+        const bool b0 = (q & 1) == 0;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const double own = b0 ? w[2 * m] : w[2 * m + 1];
+            const double send = b0 ? w[2 * m + 1] : w[2 * m];
+            S[m] = own + dpp_f64<kDppXor1>(send);
+        }
+    } else {
+        // partner q ^ 1: lane 0 keeps the even components, lane 1 the odd ones
+        const bool b0 = (q & 1) == 0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double own = b0 ? v[2 * m] : v[2 * m + 1];
+            const double send = b0 ? v[2 * m + 1] : v[2 * m];
+            S[m] = own + dpp_f64<kDppXor1>(send);
+        }
+    }
+}
+
+// Maximum over the lanes owning distinct components (for G = 16 both halves
+// hold the same values and reach the same maximum).
+template <int G>
+__device__ __forceinline__ double owners_max(double v) {
+    constexpr int Gc = MG<G>::Gc;
+    v = fmax(v, dpp_f64<kDppXor1>(v));
+    if constexpr (Gc >= 4) v = fmax(v, dpp_f64<kDppXor2>(v));
+    if constexpr (Gc >= 8) v = fmax(v, dpp_f64<kDppHalfMirror>(v));
+    return v;
+}
+
+// Component q + Gc m of a whole state vector y (select chain: the index
+// depends on the lane).
+template <int G, int m>
+__device__ __forceinline__ double own_component(const double y[8], uint32_t q) {
+    constexpr int Gc = MG<G>::Gc;
+    double v = y[Gc * m];
+#pragma unroll
+    for (int j = 1; j < Gc; ++j) v = (q == static_cast<uint32_t>(j)) ? y[Gc * m + j] : v;
+    return v;
+}
+
+// The components q + Gc m (m < C) of a whole state vector y.
+template <int G>
+__device__ __forceinline__ void own_components(const double y[8], uint32_t q, double* out) {
+    constexpr int C = MG<G>::C;
+    out[0] = own_component<G, 0>(y, q);
+    if constexpr (C > 1) out[1] = own_component<G, 1>(y, q);
+    if constexpr (C > 2) {
+        out[2] = own_component<G, 2>(y, q);
+        out[3] = own_component<G, 3>(y, q);
+    }
+}
+
 __device__ __forceinline__ void to_local(const SourceDev& s, double x, double y, double z, double l[3]) {
     const double dx = x - s.c[0], dy = y - s.c[1], dz = z - s.c[2];
     l[0] = dx * s.fx[0] + dy * s.fx[1] + dz * s.fx[2];
@@ -773,43 +923,43 @@ struct GroupSrc {
     bool has;
 };
 
-// Right-hand side of the superposed Hamiltonian (ms_rhs,
-// software_compute_engine.hpp:2857-2870, with a carried p_t): flat part plus
-// the rotated KS terms of every source; each lane adds its sources and the
-// group sums.
-template <int G>
-__device__ __forceinline__ void rhs_group(const GroupSrc& gs, const double y[8], double d[8]) {
+// One lane's source in the superposed Hamiltonian (ms_rhs,
+// software_compute_engine.hpp:2857-2870, with a carried p_t) at world
+// position X, p_t and world momentum Pm: v = (2 H s, -2 H s k, 0, dp) with
+// k and dp rotated to world axes; zero for a lane without a source.
+__device__ __forceinline__ void source_terms(const GroupSrc& gs, const double X[3], double pt, const double Pm[3],
+                                             double v[8]) {
     double ths = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0;
     if (gs.has) {
         const SourceDev* V = gs.S;
-        const double dx = y[1] - V->c[0], dy = y[2] - V->c[1], dz = y[3] - V->c[2];
+        const double dx = X[0] - V->c[0], dy = X[1] - V->c[1], dz = X[2] - V->c[2];
         double lx, ly, lz, px, py, pz;
         {
             const double f0 = V->fx[0], f1 = V->fx[1], f2 = V->fx[2];
             lx = dx * f0 + dy * f1 + dz * f2;
-            px = y[5] * f0 + y[6] * f1 + y[7] * f2;
+            px = Pm[0] * f0 + Pm[1] * f1 + Pm[2] * f2;
         }
         {
             const double f0 = V->fy[0], f1 = V->fy[1], f2 = V->fy[2];
             ly = dx * f0 + dy * f1 + dz * f2;
-            py = y[5] * f0 + y[6] * f1 + y[7] * f2;
+            py = Pm[0] * f0 + Pm[1] * f1 + Pm[2] * f2;
         }
         {
             const double f0 = V->ax[0], f1 = V->ax[1], f2 = V->ax[2];
             lz = dx * f0 + dy * f1 + dz * f2;
-            pz = y[5] * f0 + y[6] * f1 + y[7] * f2;
+            pz = Pm[0] * f0 + Pm[1] * f1 + Pm[2] * f2;
         }
         double t, c[3], q[3];
-        ks_terms<double>(lx, ly, lz, y[4], px, py, pz, kerr_const(V->m, V->a, V->a2), t, c, q);
-        ths += t;
+        ks_terms<double>(lx, ly, lz, pt, px, py, pz, kerr_const(V->m, V->a, V->a2), t, c, q);
+        ths = t;
         {
             const double f0 = V->fx[0], f1 = V->fx[1], f2 = V->fx[2];
-            c0 += c[0] * f0;
-            c1 += c[0] * f1;
-            c2 += c[0] * f2;
-            q0 += q[0] * f0;
-            q1 += q[0] * f1;
-            q2 += q[0] * f2;
+            c0 = c[0] * f0;
+            c1 = c[0] * f1;
+            c2 = c[0] * f2;
+            q0 = q[0] * f0;
+            q1 = q[0] * f1;
+            q2 = q[0] * f2;
         }
         {
             const double f0 = V->fy[0], f1 = V->fy[1], f2 = V->fy[2];
@@ -830,14 +980,84 @@ __device__ __forceinline__ void rhs_group(const GroupSrc& gs, const double y[8],
             q2 += q[2] * f2;
         }
     }
-    d[0] = -y[4] + group_sum<G>(ths);
-    d[1] = y[5] + group_sum<G>(c0);
-    d[2] = y[6] + group_sum<G>(c1);
-    d[3] = y[7] + group_sum<G>(c2);
-    d[4] = 0.0;
-    d[5] = group_sum<G>(q0);
-    d[6] = group_sum<G>(q1);
-    d[7] = group_sum<G>(q2);
+    v[0] = ths;
+    v[1] = c0;
+    v[2] = c1;
+    v[3] = c2;
+    v[4] = 0.0;
+    v[5] = q0;
+    v[6] = q1;
+    v[7] = q2;
+}
+
+// Superposed right-hand side of the group-distributed state Y (p_t given):
+// gathers the position X and momentum Pm (returned; the same in every lane),
+// adds the lane's source and reduces onto the owners, d[m] being the rate of
+// component q + Gc m: t' = -p_t + sum 2Hs, x' = p - sum 2Hs k, p_t' = 0,
+// p' = sum dp.
+template <int G>
+__device__ __forceinline__ void mrhs(const GroupSrc& gs, const MCtx& cx, const double* Y, double pt, double* d,
+                                     double X[3], double Pm[3]) {
+    constexpr int Gc = MG<G>::Gc, C = MG<G>::C;
+    X[0] = from_owner<G, 1>(Y, cx);
+    X[1] = from_owner<G, 2>(Y, cx);
+    X[2] = from_owner<G, 3>(Y, cx);
+    Pm[0] = from_owner<G, 5>(Y, cx);
+    Pm[1] = from_owner<G, 6>(Y, cx);
+    Pm[2] = from_owner<G, 7>(Y, cx);
+    double v[8];
+    source_terms(gs, X, pt, Pm, v);
+    double S[C];
+    reduce_scatter<G>(v, cx.q, S);
+#pragma unroll
+    for (int m = 0; m < C; ++m) {
+        if (Gc * m < 4) {
+            const uint32_t c = cx.q + Gc * m;
+            double base = 0.0;
+            base = (c == 1) ? Pm[0] : base;
+            base = (c == 2) ? Pm[1] : base;
+            base = (c == 3) ? Pm[2] : base;
+            base = (c == 0) ? -pt : base;
+            d[m] = base + S[m];
+        } else {
+            d[m] = S[m];
+        }
+    }
+}
+
+// The seven stages of one DP45 attempt on the group's components (as
+// dp45_stages for one hole; WithDense adds the dense-output combination).
+// X5, P5: position and momentum of y5 in every lane.
+template <bool WithDense, int G>
+__device__ __forceinline__ void mstages(const GroupSrc& gs, const MCtx& cx, const double* y, const double* k1,
+                                        double pt, double h, double* y5, double* ep, double* k7, double* dn,
+                                        double X5[3], double P5[3]) {
+    constexpr int C = MG<G>::C;
+    double k2[C], k3[C], k4[C], k5[C], k6[C], Y[C], X[3], Pm[3];
+#pragma unroll
+    for (int i = 0; i < C; ++i) Y[i] = y[i] + h * (dp::a21 * k1[i]);
+    mrhs<G>(gs, cx, Y, pt, k2, X, Pm);
+#pragma unroll
+    for (int i = 0; i < C; ++i) Y[i] = y[i] + h * (dp::a31 * k1[i] + dp::a32 * k2[i]);
+    mrhs<G>(gs, cx, Y, pt, k3, X, Pm);
+#pragma unroll
+    for (int i = 0; i < C; ++i) Y[i] = y[i] + h * (dp::a41 * k1[i] + dp::a42 * k2[i] + dp::a43 * k3[i]);
+    mrhs<G>(gs, cx, Y, pt, k4, X, Pm);
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+        Y[i] = y[i] + h * (dp::a51 * k1[i] + dp::a52 * k2[i] + dp::a53 * k3[i] + dp::a54 * k4[i]);
+    mrhs<G>(gs, cx, Y, pt, k5, X, Pm);
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+        Y[i] = y[i] + h * (dp::a61 * k1[i] + dp::a62 * k2[i] + dp::a63 * k3[i] + dp::a64 * k4[i] + dp::a65 * k5[i]);
+    mrhs<G>(gs, cx, Y, pt, k6, X, Pm);
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        y5[i] = y[i] + h * (dp::a71 * k1[i] + dp::a73 * k3[i] + dp::a74 * k4[i] + dp::a75 * k5[i] + dp::a76 * k6[i]);
+        ep[i] = dp::e1 * k1[i] + dp::e3 * k3[i] + dp::e4 * k4[i] + dp::e5 * k5[i] + dp::e6 * k6[i];
+        if (WithDense) dn[i] = dp::d1 * k1[i] + dp::d3 * k3[i] + dp::d4 * k4[i] + dp::d5 * k5[i] + dp::d6 * k6[i];
+    }
+    mrhs<G>(gs, cx, y5, pt, k7, X5, P5);
 }
 
 // Local coordinates of a world point in the frame of source V.
@@ -848,18 +1068,30 @@ __device__ __forceinline__ void to_local_v(const SourceDev* V, double x, double 
     l[2] = dx * V->ax[0] + dy * V->ax[1] + dz * V->ax[2];
 }
 
-// Superposed null constraint as a quadratic in p_t; projection as for one
-// hole. Counts the projection in L.
+// A superposition ray in its group: own components of the state and of its
+// derivative (first-same-as-last), and what every lane keeps.
+template <int C>
+struct MLane {
+    double y[C], k1[C];
+    double xs[3], ps[3], pt;  // position, momentum and p_t of the state
+    double ah, lambda, g;
+    uint32_t acc, rej, flags, cls;
+    uint32_t rhs, proj, f32;
+};
+
+// Superposed null constraint as a quadratic in p_t at (X, pt, Pm); projection
+// as for one hole. Updates pt; true if it fired.
 template <int G>
-__device__ __forceinline__ bool project_group(const GroupSrc& gs, double y[8], double tol, Lane& L) {
+__device__ __forceinline__ bool mproject(const GroupSrc& gs, const double X[3], const double Pm[3], double& pt,
+                                         double tol) {
     double h = 0.0, hq = 0.0, hq2 = 0.0;
     if (gs.has) {
         const SourceDev* V = gs.S;
         double l[3];
-        to_local_v(V, y[1], y[2], y[3], l);
-        const double px = y[5] * V->fx[0] + y[6] * V->fx[1] + y[7] * V->fx[2];
-        const double py = y[5] * V->fy[0] + y[6] * V->fy[1] + y[7] * V->fy[2];
-        const double pz = y[5] * V->ax[0] + y[6] * V->ax[1] + y[7] * V->ax[2];
+        to_local_v(V, X[0], X[1], X[2], l);
+        const double px = Pm[0] * V->fx[0] + Pm[1] * V->fx[1] + Pm[2] * V->fx[2];
+        const double py = Pm[0] * V->fy[0] + Pm[1] * V->fy[1] + Pm[2] * V->fy[2];
+        const double pz = Pm[0] * V->ax[0] + Pm[1] * V->ax[1] + Pm[2] * V->ax[2];
         const Field f = ks_field(l[0], l[1], l[2], V->m, V->a, V->a2);
         const double q = f.k0 * px + f.k1 * py + f.k2 * pz;
         h += f.H;
@@ -867,22 +1099,28 @@ __device__ __forceinline__ bool project_group(const GroupSrc& gs, double y[8], d
         hq2 += f.H * q * q;
     }
     const double sh = group_sum<G>(h), shq = group_sum<G>(hq), shq2 = group_sum<G>(hq2);
-    const double P2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
+    const double P2 = Pm[0] * Pm[0] + Pm[1] * Pm[1] + Pm[2] * Pm[2];
     const double A = -(1.0 + 2.0 * sh);
     const double B = 4.0 * shq;
     const double C0 = P2 - 2.0 * shq2;
-    const double res = (A * y[4] + B) * y[4] + C0;
-    const double scale = fmax(1.0, y[4] * y[4] + P2);
+    const double res = (A * pt + B) * pt + C0;
+    const double scale = fmax(1.0, pt * pt + P2);
     if (!(fabs(res) >= tol * scale)) return false;
     const double disc = B * B - 4.0 * A * C0;
     if (disc < 0.0) return false;
     const double sd = sqrt(disc);
     const double r1 = (-B + sd) / (2.0 * A);
     const double r2 = (-B - sd) / (2.0 * A);
-    y[4] = (fabs(r1 - y[4]) <= fabs(r2 - y[4])) ? r1 : r2;
-    L.flags |= RT_FLAG_PROJECTED;
-    ++L.proj;
+    pt = (fabs(r1 - pt) <= fabs(r2 - pt)) ? r1 : r2;
     return true;
+}
+
+// Keep the owner's copy of component 4 equal to p_t after a projection.
+template <int G>
+__device__ __forceinline__ void sync_pt(const MCtx& cx, double* y, double pt) {
+    constexpr int Gc = MG<G>::Gc;
+    constexpr int m = 4 / Gc;
+    if (cx.q == static_cast<uint32_t>(4 % Gc)) y[m] = pt;
 }
 
 // Camera pixel -> initial state in a superposition (trace_multi_source set-up,
@@ -891,7 +1129,7 @@ __device__ __forceinline__ bool project_group(const GroupSrc& gs, double y[8], d
 __device__ __forceinline__ void camera_state_multi(const TraceParams& P, const MultiDev& T, uint32_t px, uint32_t py,
                                                    double y[8]) {
     double o[3];
-    camera_outward(P, px, py, o);
+    camera_outward<true>(P, px, py, o);
     double vv[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) vv[c] = (c == 0 ? T.u0 : 0.0) - o[0] * T.e[0][c] - o[1] * T.e[1][c] - o[2] * T.e[2][c];
@@ -919,13 +1157,21 @@ __device__ __forceinline__ void camera_state_multi(const TraceParams& P, const M
 // A ray through a superposition (FP64): DP45 as for one hole; events per
 // source (horizon of its local KS radius, crossing of its local equatorial
 // plane inside its disk), escape on the Euclidean radius while outgoing.
-// multi_begin sets a group up, multi_step makes one attempt and returns true
-// (in every lane of the group) once the ray has terminated.
+// mbegin sets a group up from the whole initial state y0 (every lane), mstep
+// makes one attempt and returns true (in every lane of the group) once the
+// ray has terminated.
 template <int G>
-__device__ __forceinline__ void multi_begin(const TraceParams& P, const GroupSrc& gs, const double y0[8], Lane& L) {
+__device__ __forceinline__ void mbegin(const TraceParams& P, const GroupSrc& gs, const MCtx& cx, const double y0[8],
+                                       MLane<MG<G>::C>& L) {
+    own_components<G>(y0, cx.q, L.y);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
-    rhs_group<G>(gs, L.y, L.k1);
+    for (int i = 0; i < 3; ++i) {
+        L.xs[i] = y0[1 + i];
+        L.ps[i] = y0[5 + i];
+    }
+    L.pt = y0[4];
+    double X[3], Pm[3];
+    mrhs<G>(gs, cx, L.y, L.pt, L.k1, X, Pm);
     L.rhs = 1;
     L.proj = L.f32 = 0;
     L.acc = L.rej = L.flags = L.cls = 0;
@@ -935,7 +1181,7 @@ __device__ __forceinline__ void multi_begin(const TraceParams& P, const GroupSrc
     if (gs.has) {
         const SourceDev* V = gs.S;
         double l[3];
-        to_local_v(V, L.y[1], L.y[2], L.y[3], l);
+        to_local_v(V, L.xs[0], L.xs[1], L.xs[2], l);
         const double r = ks_radius(l[0], l[1], l[2], V->a2);
         const double rh = V->rh;
         scale = fmin(scale, sqrt(r * fmax(r - rh, 0.02 * V->m)));
@@ -947,69 +1193,59 @@ __device__ __forceinline__ void multi_begin(const TraceParams& P, const GroupSrc
     L.ah = fmin(fmax(ah, P.h_min), P.h_max);
 }
 
-// The seven stages of one DP45 attempt on the group right-hand side (as
-// dp45_stages for one hole; WithDense adds the dense-output combination).
-template <bool WithDense, int G>
-__device__ __forceinline__ void dp45_stages_group(const GroupSrc& gs, const double y[8], const double k1[8], double h,
-                                                  double y5[8], double ep[8], double k7[8], double dn[8]) {
-    double k2[8], k3[8], k4[8], k5[8], k6[8], Y[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a21 * k1[i]);
-    rhs_group<G>(gs, Y, k2);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a31 * k1[i] + dp::a32 * k2[i]);
-    rhs_group<G>(gs, Y, k3);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) Y[i] = y[i] + h * (dp::a41 * k1[i] + dp::a42 * k2[i] + dp::a43 * k3[i]);
-    rhs_group<G>(gs, Y, k4);
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-        Y[i] = y[i] + h * (dp::a51 * k1[i] + dp::a52 * k2[i] + dp::a53 * k3[i] + dp::a54 * k4[i]);
-    rhs_group<G>(gs, Y, k5);
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-        Y[i] = y[i] + h * (dp::a61 * k1[i] + dp::a62 * k2[i] + dp::a63 * k3[i] + dp::a64 * k4[i] + dp::a65 * k5[i]);
-    rhs_group<G>(gs, Y, k6);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        y5[i] = y[i] + h * (dp::a71 * k1[i] + dp::a73 * k3[i] + dp::a74 * k4[i] + dp::a75 * k5[i] + dp::a76 * k6[i]);
-        ep[i] = dp::e1 * k1[i] + dp::e3 * k3[i] + dp::e4 * k4[i] + dp::e5 * k5[i] + dp::e6 * k6[i];
-        if (WithDense) dn[i] = dp::d1 * k1[i] + dp::d3 * k3[i] + dp::d4 * k4[i] + dp::d5 * k5[i] + dp::d6 * k6[i];
-    }
-    rhs_group<G>(gs, y5, k7);
+// Dense-output coefficients of component c (compile time), in every lane.
+template <int G, int c>
+__device__ __forceinline__ Dense dense_from_owner(const double* D, const double* B, const double* Cc, const double* E,
+                                                  const MCtx& cx) {
+    Dense q;
+    q.D = from_owner<G, c>(D, cx);
+    q.B = from_owner<G, c>(B, cx);
+    q.C = from_owner<G, c>(Cc, cx);
+    q.E = from_owner<G, c>(E, cx);
+    return q;
 }
 
 // Event location inside an accepted step [y, y5] of size h: every lane the
 // horizons and disks of its own sources, then the group's earliest (ties to
 // the lower source index, as a scan over the sources in order keeps them),
 // then escape on the Euclidean radius while outgoing. Recomputes the step
-// with the dense-output combination. Fills the end state; returns the class
-// (0: no event), the same in every lane of the group.
+// with the dense-output combination; the dense coefficients of the position
+// and momentum are gathered into every lane. Fills the end state (own
+// components ye; position Xe and momentum Pe in every lane); returns the
+// class (0: no event), the same in every lane of the group.
 template <int G>
-__device__ __forceinline__ uint32_t locate_event_group(const TraceParams& P, const GroupSrc& gs, uint32_t si,
-                                                       uint32_t lane0, const double y[8], const double k1[8], double h, bool es,
-                                                       double yend[8], double* s_out, double* g_out) {
-    double y5[8], ep[8], k7[8], dn[8];
-    dp45_stages_group<true, G>(gs, y, k1, h, y5, ep, k7, dn);
-    auto coef = [&](int i) {
-        Dense q;
-        q.D = y5[i] - y[i];
-        q.B = h * k1[i] - q.D;
-        q.C = q.D - h * k7[i] - q.B;
-        q.E = h * (dn[i] + dp::d7 * k7[i]);
-        return q;
-    };
-    const Dense qx = coef(1), qy = coef(2), qz = coef(3);
+__device__ __forceinline__ uint32_t mlocate_event(const TraceParams& P, const GroupSrc& gs, const MCtx& cx,
+                                                  uint32_t si, uint32_t lane0, const MLane<MG<G>::C>& L, double h,
+                                                  bool es, double* ye, double Xe[3], double Pe[3], double* s_out,
+                                                  double* g_out) {
+    constexpr int C = MG<G>::C;
+    double y5[C], ep[C], k7[C], dn[C], X5[3], P5[3];
+    mstages<true, G>(gs, cx, L.y, L.k1, L.pt, h, y5, ep, k7, dn, X5, P5);
+    double qD[C], qB[C], qC[C], qE[C];
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        qD[i] = y5[i] - L.y[i];
+        qB[i] = h * L.k1[i] - qD[i];
+        qC[i] = qD[i] - h * k7[i] - qB[i];
+        qE[i] = h * (dn[i] + dp::d7 * k7[i]);
+    }
+    const Dense qx = dense_from_owner<G, 1>(qD, qB, qC, qE, cx);
+    const Dense qy = dense_from_owner<G, 2>(qD, qB, qC, qE, cx);
+    const Dense qz = dense_from_owner<G, 3>(qD, qB, qC, qE, cx);
+    const Dense q5 = dense_from_owner<G, 5>(qD, qB, qC, qE, cx);
+    const Dense q6 = dense_from_owner<G, 6>(qD, qB, qC, qE, cx);
+    const Dense q7 = dense_from_owner<G, 7>(qD, qB, qC, qE, cx);
+    const double* y = L.xs;
     double best = 2.0, best_g = 0.0;
     uint32_t best_cls = 0, best_src = 0xFFFFFFFFu;
     if (gs.has) {
         const SourceDev* V = gs.S;
         auto local_at = [&](double ss, double l[3]) {
-            to_local_v(V, dense_at(qx, y[1], ss), dense_at(qy, y[2], ss), dense_at(qz, y[3], ss), l);
+            to_local_v(V, dense_at(qx, y[0], ss), dense_at(qy, y[1], ss), dense_at(qz, y[2], ss), l);
         };
         double l0[3], l1[3];
-        to_local_v(V, y[1], y[2], y[3], l0);
-        to_local_v(V, y5[1], y5[2], y5[3], l1);
+        to_local_v(V, y[0], y[1], y[2], l0);
+        to_local_v(V, X5[0], X5[1], X5[2], l1);
         const double a2 = V->a2, r_hz = V->r_hz;
         const double r0 = ks_radius(l0[0], l0[1], l0[2], a2);
         const double r1 = ks_radius(l1[0], l1[1], l1[2], a2);
@@ -1040,8 +1276,8 @@ __device__ __forceinline__ uint32_t locate_event_group(const TraceParams& P, con
             local_at(sd, l);
             const double rc = ks_radius(l[0], l[1], l[2], a2);
             if (rc >= V->disk_in && rc <= V->disk_out && rc > V->rh102 && sd < best) {
-                const Dense q5 = coef(5), q6 = coef(6), q7 = coef(7);
-                const double pw0 = dense_at(q5, y[5], sd), pw1 = dense_at(q6, y[6], sd), pw2 = dense_at(q7, y[7], sd);
+                const double pw0 = dense_at(q5, L.ps[0], sd), pw1 = dense_at(q6, L.ps[1], sd),
+                             pw2 = dense_at(q7, L.ps[2], sd);
                 const double pl0 = pw0 * V->fx[0] + pw1 * V->fx[1] + pw2 * V->fx[2];
                 const double pl1 = pw0 * V->fy[0] + pw1 * V->fy[1] + pw2 * V->fy[2];
                 const double pl2 = pw0 * V->ax[0] + pw1 * V->ax[1] + pw2 * V->ax[2];
@@ -1054,7 +1290,7 @@ __device__ __forceinline__ uint32_t locate_event_group(const TraceParams& P, con
             }
         }
     }
-    if (G > 1) {
+    {
         uint32_t win = best_src;
         group_argmin<G>(best, win);
         if (best < 2.0) {
@@ -1065,13 +1301,13 @@ __device__ __forceinline__ uint32_t locate_event_group(const TraceParams& P, con
     }
     if (!(best < 2.0)) best_cls = 0;
     if (es) {
-        const double R0 = sqrt(y[1] * y[1] + y[2] * y[2] + y[3] * y[3]);
-        const double R1 = sqrt(y5[1] * y5[1] + y5[2] * y5[2] + y5[3] * y5[3]);
+        const double R0 = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        const double R1 = sqrt(X5[0] * X5[0] + X5[1] * X5[1] + X5[2] * X5[2]);
         double se = 1.0;
         if (R0 < P.r_esc)
             se = illinois_root(
                 [&](double ss) {
-                    const double x = dense_at(qx, y[1], ss), yy = dense_at(qy, y[2], ss), z = dense_at(qz, y[3], ss);
+                    const double x = dense_at(qx, y[0], ss), yy = dense_at(qy, y[1], ss), z = dense_at(qz, y[2], ss);
                     return sqrt(x * x + yy * yy + z * z) - P.r_esc;
                 },
                 R0 - P.r_esc, R1 - P.r_esc);
@@ -1083,10 +1319,28 @@ __device__ __forceinline__ uint32_t locate_event_group(const TraceParams& P, con
     if (best_cls == 0) return 0;
     if (best >= 1.0) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) yend[i] = y5[i];
+        for (int i = 0; i < C; ++i) ye[i] = y5[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            Xe[i] = X5[i];
+            Pe[i] = P5[i];
+        }
     } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) yend[i] = dense_at(coef(i), y[i], best);
+        for (int i = 0; i < C; ++i) {
+            Dense q;
+            q.D = qD[i];
+            q.B = qB[i];
+            q.C = qC[i];
+            q.E = qE[i];
+            ye[i] = dense_at(q, L.y[i], best);
+        }
+        Xe[0] = dense_at(qx, y[0], best);
+        Xe[1] = dense_at(qy, y[1], best);
+        Xe[2] = dense_at(qz, y[2], best);
+        Pe[0] = dense_at(q5, L.ps[0], best);
+        Pe[1] = dense_at(q6, L.ps[1], best);
+        Pe[2] = dense_at(q7, L.ps[2], best);
     }
     *s_out = best;
     *g_out = best_g;
@@ -1094,34 +1348,42 @@ __device__ __forceinline__ uint32_t locate_event_group(const TraceParams& P, con
 }
 
 template <int G>
-__device__ __forceinline__ bool multi_step(const TraceParams& P, const GroupSrc& gs, uint32_t si, uint32_t lane0,
-                                           Lane& L) {
+__device__ __forceinline__ bool mstep(const TraceParams& P, const GroupSrc& gs, const MCtx& cx, uint32_t si,
+                                      uint32_t lane0, MLane<MG<G>::C>& L) {
+    constexpr int Gc = MG<G>::Gc, C = MG<G>::C;
     if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
         L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
         double near = 0.0;
         if (gs.has) {
             const SourceDev* V = gs.S;
-            const double dx = L.y[1] - V->c[0], dy = L.y[2] - V->c[1], dz = L.y[3] - V->c[2];
+            const double dx = L.xs[0] - V->c[0], dy = L.xs[1] - V->c[1], dz = L.xs[2] - V->c[2];
             if (sqrt(dx * dx + dy * dy + dz * dz) < 6.0 * V->m) near = 1.0;
         }
         L.cls = (group_min<G>(-near) < 0.0) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
         return true;
     }
     const double ah = L.ah, h = -ah;
-    double y5[8], ep[8], k7[8];
-    dp45_stages_group<false, G>(gs, L.y, L.k1, h, y5, ep, k7, nullptr);
+    double y5[C], ep[C], k7[C], X5[3], P5[3];
+    mstages<false, G>(gs, cx, L.y, L.k1, L.pt, h, y5, ep, k7, nullptr, X5, P5);
     L.rhs += 6;
+    // err = |h| max_i |sum_j e_j k_j,i| / (rtol max(|y_i|, |y5_i|) + atol) over the
+    // components other than p_t; infinite when a component is not finite
     double err = 0.0;
-    bool finite = true;
+    bool ok = true;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        finite = finite && isfinite(y5[i]);
-        if (i == 4) continue;
-        const double e = ah * fabs(ep[i] + dp::e7 * k7[i]);
-        const double sc = P.rtol * fmax(fabs(L.y[i]), fabs(y5[i])) + P.atol;
-        err = fmax(err, e * rcp64(sc));
+    for (int m = 0; m < C; ++m) {
+        const uint32_t c = cx.q + Gc * m;
+        const double e = ah * fabs(ep[m] + dp::e7 * k7[m]);
+        const double sc = P.rtol * fmax(fabs(L.y[m]), fabs(y5[m])) + P.atol;
+        const double r = e * rcp64(sc);
+        ok = ok && isfinite(y5[m]);
+        if (c != 4) {
+            ok = ok && isfinite(r);
+            err = fmax(err, r);
+        }
     }
-    if (!finite || !isfinite(err)) {
+    err = owners_max<G>(ok ? err : __builtin_huge_val());
+    if (!(err < __builtin_huge_val())) {
         L.flags |= RT_FLAG_NONFINITE;
         L.cls = RT_CLS_CAPTURED;
         return true;
@@ -1133,28 +1395,41 @@ __device__ __forceinline__ bool multi_step(const TraceParams& P, const GroupSrc&
     }
     ++L.acc;
     // any event in this step? (a horizon reached or a disk plane crossed by one
-    // of the lane's sources, escape)
+    // of the group's sources, escape)
     double mine = 0.0;
     if (gs.has) {
         const SourceDev* V = gs.S;
         double l0[3], l1[3];
-        to_local_v(V, L.y[1], L.y[2], L.y[3], l0);
-        to_local_v(V, y5[1], y5[2], y5[3], l1);
+        to_local_v(V, L.xs[0], L.xs[1], L.xs[2], l0);
+        to_local_v(V, X5[0], X5[1], X5[2], l1);
         const bool hz = ks_radius(l1[0], l1[1], l1[2], V->a2) <= V->r_hz;
         const bool dk = V->disk_on && ((l0[2] < 0.0 && l1[2] >= 0.0) || (l0[2] > 0.0 && l1[2] <= 0.0));
         if (hz || dk) mine = 1.0;
     }
     const bool src_ev = group_min<G>(-mine) < 0.0;
-    const double R1 = sqrt(y5[1] * y5[1] + y5[2] * y5[2] + y5[3] * y5[3]);
-    const bool es = (R1 >= P.r_esc) && (-(y5[1] * k7[1] + y5[2] * k7[2] + y5[3] * k7[3]) > 0.0);
+    const double R1 = sqrt(X5[0] * X5[0] + X5[1] * X5[1] + X5[2] * X5[2]);
+    bool es = false;
+    if (R1 >= P.r_esc) {
+        const double kx = from_owner<G, 1>(k7, cx), ky = from_owner<G, 2>(k7, cx), kz = from_owner<G, 3>(k7, cx);
+        es = -(X5[0] * kx + X5[1] * ky + X5[2] * kz) > 0.0;
+    }
     if (src_ev || es) {
-        double s_ev = 1.0, g_ev = 0.0, ye[8];
-        const uint32_t c = locate_event_group<G>(P, gs, si, lane0, L.y, L.k1, h, es, ye, &s_ev, &g_ev);
+        double s_ev = 1.0, g_ev = 0.0, ye[C], Xe[3], Pe[3];
+        const uint32_t c = mlocate_event<G>(P, gs, cx, si, lane0, L, h, es, ye, Xe, Pe, &s_ev, &g_ev);
         if (c != 0) {
             L.rhs += 6;
-            project_group<G>(gs, ye, P.inv_tol, L);
+            if (mproject<G>(gs, Xe, Pe, L.pt, P.inv_tol)) {
+                L.flags |= RT_FLAG_PROJECTED;
+                ++L.proj;
+            }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) L.y[i] = ye[i];
+            for (int i = 0; i < C; ++i) L.y[i] = ye[i];
+            sync_pt<G>(cx, L.y, L.pt);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                L.xs[i] = Xe[i];
+                L.ps[i] = Pe[i];
+            }
             L.lambda += s_ev * ah;
             L.cls = c;
             L.g = g_ev;
@@ -1162,52 +1437,88 @@ __device__ __forceinline__ bool multi_step(const TraceParams& P, const GroupSrc&
         }
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) L.y[i] = y5[i];
+    for (int i = 0; i < C; ++i) L.y[i] = y5[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        L.xs[i] = X5[i];
+        L.ps[i] = P5[i];
+    }
     L.lambda += ah;
-    if (project_group<G>(gs, L.y, P.inv_tol, L)) {
-        rhs_group<G>(gs, L.y, L.k1);
+    if (mproject<G>(gs, L.xs, L.ps, L.pt, P.inv_tol)) {
+        L.flags |= RT_FLAG_PROJECTED;
+        ++L.proj;
+        sync_pt<G>(cx, L.y, L.pt);
+        double X[3], Pm[3];
+        mrhs<G>(gs, cx, L.y, L.pt, L.k1, X, Pm);
         ++L.rhs;
     } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) L.k1[i] = k7[i];
+        for (int i = 0; i < C; ++i) L.k1[i] = k7[i];
     }
     L.ah = next_step(ah, err, true, P.h_min, P.h_max);
     return false;
 }
 
-// Store with BL coordinates taken in the frame of source 0 (the central hole).
-__device__ __forceinline__ void store_multi(const OutPtrs& O, uint64_t idx, const Lane& L, const MultiDev& T) {
+// Redshift per class and the final flag word of a superposition ray.
+template <int C>
+__device__ __forceinline__ void mlane_finish(MLane<C>& L) {
+    const uint32_t c = L.cls & RT_CLS_MASK;
+    if (c == RT_CLS_ESCAPED) L.g = 1.0 / (-L.pt);
+    if (c == RT_CLS_CAPTURED) L.g = 0.0;
+    L.cls |= L.flags & ~(kCarterNoTurn | kNoEventRetry);
+}
+
+// The per-ray outputs other than the state (not inlined, for the same reason
+// as camera_state_multi: atan2's constants).
+__device__ __noinline__ void store_multi_rest(const OutPtrs& O, uint64_t idx, uint32_t cls, double x, double y,
+                                              double z, double lambda, double g, uint32_t acc, uint32_t rej,
+                                              const SourceDev& s0) {
     const uint64_t n = O.n;
-    O.cls[idx] = L.cls;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        O.x[c * n + idx] = L.y[c];
-        O.p[c * n + idx] = L.y[4 + c];
-    }
-    double l[3], bl[3];
-    to_local(T.src[0], L.y[1], L.y[2], L.y[3], l);
-    bl_from_ks(l[0], l[1], l[2], T.src[0].a, T.src[0].a2, bl);
+    O.cls[idx] = cls;
+    double lc[3], bl[3];
+    to_local(s0, x, y, z, lc);
+    bl_from_ks(lc[0], lc[1], lc[2], s0.a, s0.a2, bl);
 #pragma unroll
     for (int c = 0; c < 3; ++c) O.bl[c * n + idx] = bl[c];
-    O.lambda[idx] = L.lambda;
-    O.g[idx] = L.g;
-    O.steps[idx] = L.acc;
-    O.rejected[idx] = L.rej;
+    O.lambda[idx] = lambda;
+    O.g[idx] = g;
+    O.steps[idx] = acc;
+    O.rejected[idx] = rej;
+}
+
+// Store: every owner its components (for G = 16 the lower half), lane 0 of
+// the group the rest, with BL coordinates taken in the frame of source 0 (the
+// central hole).
+template <int G>
+__device__ __forceinline__ void store_mlane(const OutPtrs& O, uint64_t idx, const MLane<MG<G>::C>& L,
+                                            const MultiDev& T, const MCtx& cx, uint32_t l) {
+    constexpr int Gc = MG<G>::Gc, C = MG<G>::C;
+    const uint64_t n = O.n;
+    if (l < static_cast<uint32_t>(Gc)) {
+#pragma unroll
+        for (int m = 0; m < C; ++m) {
+            const uint32_t c = cx.q + Gc * m;
+            double* dst = (c < 4) ? O.x : O.p;
+            dst[(c & 3) * n + idx] = L.y[m];
+        }
+    }
+    if (l == 0) store_multi_rest(O, idx, L.cls, L.xs[0], L.xs[1], L.xs[2], L.lambda, L.g, L.acc, L.rej, T.src[0]);
 }
 
 // Superposition request: a persistent kernel, G lanes per ray (G = 2, 4, 8
-// or 16, lane s of a group takes source s). Idle groups are
-// refilled from a device counter (ballot over the group leaders, one
-// atomicAdd per refill, rank by popcount), so a group whose ray terminates
-// takes the next ray while the slow rays of the wave continue. Camera rays
-// are handed out in 8x8-tile order (ray g: tile g / 64, pixel g % 64). The
-// source table is staged into LDS. *next starts at zero.
+// or 16, lane s of a group takes source s and a share of the components).
+// Idle groups are refilled from a device counter (ballot over the group
+// leaders, one atomicAdd per refill, rank by popcount), so a group whose ray
+// terminates takes the next ray while the slow rays of the wave continue.
+// Camera rays are handed out in 8x8-tile order (ray g: tile g / 64, pixel
+// g % 64). The source table is staged into LDS. *next starts at zero.
 template <int G, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPtrs O,
                                                             const MultiDev* __restrict__ Tg,
                                                             unsigned long long* __restrict__ next,
                                                             uint32_t refill_min, const uint32_t* limit,
                                                             unsigned long long* active_waves) {
+    constexpr int C = MG<G>::C;
     __shared__ MultiDev sm;
     {
         const uint32_t words = sizeof(MultiDev) / 8;
@@ -1222,9 +1533,10 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
     const uint32_t lane0 = lane_id & ~static_cast<uint32_t>(G - 1);
     const uint64_t below = (lane0 == 0) ? 0ull : (~0ull >> (64 - lane0));
     const GroupSrc gs{&sm.src[si < T.n ? si : 0], si < T.n};
+    const MCtx cx{si & static_cast<uint32_t>(MG<G>::Gc - 1), static_cast<int>(4 * lane0)};
     const uint64_t total = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.cols + 7) / 8) * 64
                                          : static_cast<uint64_t>(P.n_rays);
-    Lane L;
+    MLane<C> L;
     bool active = false, drained = false;
     uint64_t idx = 0;
     Tally tally;
@@ -1268,16 +1580,21 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
                     }
                     if (ok && P.mode == 0 && T.observer_inside) {
                         // observer inside a horizon: captured where it stands
+                        own_components<G>(y0, cx.q, L.y);
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) L.y[i] = y0[i];
+                        for (int i = 0; i < 3; ++i) {
+                            L.xs[i] = y0[1 + i];
+                            L.ps[i] = y0[5 + i];
+                        }
+                        L.pt = y0[4];
                         L.acc = L.rej = L.flags = L.proj = L.f32 = L.rhs = 0;
                         L.lambda = 0.0;
                         L.g = 0.0;
                         L.cls = RT_CLS_CAPTURED;
-                        lane_finish(L);
-                        if (si == 0) store_multi(O, idx, L, T);
+                        mlane_finish(L);
+                        store_mlane<G>(O, idx, L, T, cx, si);
                     } else if (ok) {
-                        multi_begin<G>(P, gs, y0, L);
+                        mbegin<G>(P, gs, cx, y0, L);
                         active = true;
                     }
                 }
@@ -1287,12 +1604,10 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
             if (drained) break;
             continue;
         }
-        if (active && multi_step<G>(P, gs, si, lane0, L)) {
-            lane_finish(L);
-            if (si == 0) {
-                store_multi(O, idx, L, T);
-                tally.add(L);
-            }
+        if (active && mstep<G>(P, gs, cx, si, lane0, L)) {
+            mlane_finish(L);
+            store_mlane<G>(O, idx, L, T, cx, si);
+            if (si == 0) tally.add(L);
             active = false;
         }
     }
