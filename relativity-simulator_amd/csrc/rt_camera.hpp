@@ -120,7 +120,7 @@ __host__ __device__ __forceinline__ void screen_direction(const TraceParams& P, 
 // library constants of these projections (sin, cos, atan, asin polynomials)
 // would be hoisted out of the loop and held in registers for the kernel's
 // whole life.
-static __device__ __noinline__ void screen_direction_out_of_line(const TraceParams& P, double u, double v,
+inline __device__ __noinline__ void screen_direction_out_of_line(const TraceParams& P, double u, double v,
                                                                  double nl[3]) {
     screen_direction(P, u, v, nl);
 }

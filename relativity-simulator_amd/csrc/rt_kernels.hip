@@ -684,35 +684,32 @@ __global__ __launch_bounds__(256, Waves) void k_trace_tiles(QueueDev Q, const Tr
 }
 
 // ------------------------------------------------------- superposition
-// Component-distributed layout. The G lanes of a group (G = 2, 4, 8 or 16,
-// the smallest power of two not below the number of sources) trace one ray
-// together. Lane l of the group evaluates source l, and owns the state
-// components c = q + Gc m (m < C) of the DP45 step, with q = l mod Gc,
-// Gc = min(G, 8) and C = 8 / Gc: it keeps their values, their seven stages
-// and their stage sums, so no lane holds the whole step (for G = 16 the two
-// halves of the group own the same components). Per right-hand side the
-// group gathers the position and momentum from their owners (every lane
-// needs them for its source), each lane adds its source's terms, and the
-// sums over the sources are reduced onto the owners of the components
-// (reduce-scatter by DPP butterflies, no memory). Values every lane needs
+// Component-distributed layout. The G lanes of a group (G = 2, 4 or 8; see
+// multi_ray_lanes) trace one ray together. Lane l of the group evaluates
+// source l (and source l + 8 in groups of 8), and owns the state components
+// c = q + Gc m (m < C) of the DP45 step, with q = l, Gc = G and C = 8 / G: it
+// keeps their values, their seven stages and their stage sums, so no lane
+// holds the whole step. Per right-hand side the group gathers the position
+// and momentum from their owners (every lane needs them for its sources),
+// each lane adds its sources' terms, and the sums over the sources are
+// reduced onto the owners of the components (reduce-scatter by DPP
+// butterflies, no memory). Values every lane needs
 // (p_t, the current position and momentum, the step size, counters) are kept
 // in every lane. Floating-point addition is commutative bit for bit, so both
 // partners of a butterfly level hold identical bits: every group-wide value
 // is the same in all lanes of the group, and the lanes take the same
 // accept/reject and event decisions in lock step. A ray's latency per step is
-// that of one source and one share of the stage arithmetic, which matters
-// because a superposition frame's time is set by its slowest rays (rays that
-// graze a horizon take thousands of steps).
+// that of its lane's sources and one share of the stage arithmetic, which
+// matters because a frame with few sources is bound by its slowest rays (rays
+// that graze a horizon take thousands of steps).
 
 // DPP lane moves inside rows of 16: xor 1 and xor 2 (quad permutes), mirror
-// inside 8 lanes and inside 16 lanes, rotation by 8 inside 16 lanes (= xor 8).
-// Each pairs every lane with a partner in the same aligned group of 2, 4, 8
-// or 16 lanes.
+// inside 8 lanes and inside 16 lanes. Each pairs every lane with a partner in
+// the same aligned group of 2, 4, 8 or 16 lanes.
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141; // row_half_mirror
 constexpr int kDppMirror = 0x140;     // row_mirror
-constexpr int kDppRor8 = 0x128;       // row_ror:8
 
 template <int Ctrl>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -795,8 +792,8 @@ struct MCtx {
 
 // Component c (compile time) of a group-distributed vector v, in every lane of
 // the group: read from its owner (lane q_c = c mod Gc of the group, register
-// m = c / Gc). Pairs and quads broadcast by a quad permute; groups of 8 and 16
-// read the lower half's owner through the LDS crossbar.
+// m = c / Gc). Pairs and quads broadcast by a quad permute; groups of 8 read
+// the owner through the LDS crossbar.
 template <int G, int c>
 __device__ __forceinline__ double from_owner(const double* v, const MCtx& cx) {
     constexpr int Gc = MG<G>::Gc;
@@ -816,12 +813,7 @@ __device__ __forceinline__ double from_owner(const double* v, const MCtx& cx) {
 template <int G>
 __device__ __forceinline__ void reduce_scatter(double v[8], uint32_t q, double* S) {
     constexpr int Gc = MG<G>::Gc;
-    if constexpr (G == 16) {
-        // the two halves of the row (lanes q and q + 8) then hold the same partial sums
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (c != 4) v[c] = v[c] + dpp_f64<kDppRor8>(v[c]);
-    }
+    static_assert(G == 2 || G == 4 || G == 8, "groups of 2, 4 or 8 lanes");
     if constexpr (Gc == 8) {
         // lanes 0-3 keep components 0-3, lanes 4-7 components 4-7 (partner 7 - q)
         const bool lo = q < 4;
@@ -875,8 +867,7 @@ __device__ __forceinline__ void reduce_scatter(double v[8], uint32_t q, double* 
     }
 }
 
-// Maximum over the lanes owning distinct components (for G = 16 both halves
-// hold the same values and reach the same maximum).
+// Maximum over the lanes of the group.
 template <int G>
 __device__ __forceinline__ double owners_max(double v) {
     constexpr int Gc = MG<G>::Gc;
@@ -916,22 +907,37 @@ __device__ __forceinline__ void to_local(const SourceDev& s, double x, double y,
     l[2] = dx * s.ax[0] + dy * s.ax[1] + dz * s.ax[2];
 }
 
-// The source of one lane: source si of the table in LDS (has: si is below
-// the number of sources; G is never smaller than the number of sources).
+// The sources of one lane: sources si and, in groups of 8 lanes, si + 8 of
+// the table in LDS, those that exist (si: the lane's index in its group of G).
 struct GroupSrc {
     const SourceDev* S;
-    bool has;
+    uint32_t first, n;
 };
 
-// One lane's source in the superposed Hamiltonian (ms_rhs,
+// Sources per lane in a group of G lanes (multi_ray_lanes picks G).
+template <int G>
+constexpr int lane_sources() {
+    return (G == 8) ? 2 : 1;
+}
+
+// f(V, i) for each source of the lane, in order.
+template <int G, class F>
+__device__ __forceinline__ void for_sources(const GroupSrc& gs, F&& f) {
+#pragma unroll
+    for (int k = 0; k < lane_sources<G>(); ++k) {
+        const uint32_t i = gs.first + static_cast<uint32_t>(k * G);
+        if (i < gs.n) f(gs.S + i, i);
+    }
+}
+
+// One source's terms in the superposed Hamiltonian (ms_rhs,
 // software_compute_engine.hpp:2857-2870, with a carried p_t) at world
-// position X, p_t and world momentum Pm: v = (2 H s, -2 H s k, 0, dp) with
-// k and dp rotated to world axes; zero for a lane without a source.
-__device__ __forceinline__ void source_terms(const GroupSrc& gs, const double X[3], double pt, const double Pm[3],
-                                             double v[8]) {
-    double ths = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0;
-    if (gs.has) {
-        const SourceDev* V = gs.S;
+// position X, p_t and world momentum Pm, added to v: (2 H s, -2 H s k, 0, dp)
+// with k and dp rotated to world axes.
+__device__ __forceinline__ void add_source_terms(const SourceDev* V, const double X[3], double pt, const double Pm[3],
+                                                 double v[8]) {
+    double ths, c0, c1, c2, q0, q1, q2;
+    {
         const double dx = X[0] - V->c[0], dy = X[1] - V->c[1], dz = X[2] - V->c[2];
         double lx, ly, lz, px, py, pz;
         {
@@ -980,14 +986,13 @@ __device__ __forceinline__ void source_terms(const GroupSrc& gs, const double X[
             q2 += q[2] * f2;
         }
     }
-    v[0] = ths;
-    v[1] = c0;
-    v[2] = c1;
-    v[3] = c2;
-    v[4] = 0.0;
-    v[5] = q0;
-    v[6] = q1;
-    v[7] = q2;
+    v[0] += ths;
+    v[1] += c0;
+    v[2] += c1;
+    v[3] += c2;
+    v[5] += q0;
+    v[6] += q1;
+    v[7] += q2;
 }
 
 // Superposed right-hand side of the group-distributed state Y (p_t given):
@@ -1005,8 +1010,8 @@ __device__ __forceinline__ void mrhs(const GroupSrc& gs, const MCtx& cx, const d
     Pm[0] = from_owner<G, 5>(Y, cx);
     Pm[1] = from_owner<G, 6>(Y, cx);
     Pm[2] = from_owner<G, 7>(Y, cx);
-    double v[8];
-    source_terms(gs, X, pt, Pm, v);
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for_sources<G>(gs, [&](const SourceDev* V, uint32_t) { add_source_terms(V, X, pt, Pm, v); });
     double S[C];
     reduce_scatter<G>(v, cx.q, S);
 #pragma unroll
@@ -1085,8 +1090,7 @@ template <int G>
 __device__ __forceinline__ bool mproject(const GroupSrc& gs, const double X[3], const double Pm[3], double& pt,
                                          double tol) {
     double h = 0.0, hq = 0.0, hq2 = 0.0;
-    if (gs.has) {
-        const SourceDev* V = gs.S;
+    for_sources<G>(gs, [&](const SourceDev* V, uint32_t) {
         double l[3];
         to_local_v(V, X[0], X[1], X[2], l);
         const double px = Pm[0] * V->fx[0] + Pm[1] * V->fx[1] + Pm[2] * V->fx[2];
@@ -1097,7 +1101,7 @@ __device__ __forceinline__ bool mproject(const GroupSrc& gs, const double X[3], 
         h += f.H;
         hq += f.H * q;
         hq2 += f.H * q * q;
-    }
+    });
     const double sh = group_sum<G>(h), shq = group_sum<G>(hq), shq2 = group_sum<G>(hq2);
     const double P2 = Pm[0] * Pm[0] + Pm[1] * Pm[1] + Pm[2] * Pm[2];
     const double A = -(1.0 + 2.0 * sh);
@@ -1178,15 +1182,14 @@ __device__ __forceinline__ void mbegin(const TraceParams& P, const GroupSrc& gs,
     L.lambda = 0.0;
     L.g = 0.0;
     double scale = 1e300, blend = 1.0;
-    if (gs.has) {
-        const SourceDev* V = gs.S;
+    for_sources<G>(gs, [&](const SourceDev* V, uint32_t) {
         double l[3];
         to_local_v(V, L.xs[0], L.xs[1], L.xs[2], l);
         const double r = ks_radius(l[0], l[1], l[2], V->a2);
         const double rh = V->rh;
         scale = fmin(scale, sqrt(r * fmax(r - rh, 0.02 * V->m)));
         blend = fmin(blend, fmin(fmax((r - 20.0 * rh) / (80.0 * rh), 0.0), 1.0));
-    }
+    });
     scale = group_min<G>(scale);
     blend = group_min<G>(blend);
     const double ah = (P.h0 > 0.0) ? P.h0 : fmin(fmax(0.5 * scale * (1.0 + 6.8 * blend), 0.004), 3.5 * 7.8);
@@ -1238,8 +1241,7 @@ __device__ __forceinline__ uint32_t mlocate_event(const TraceParams& P, const Gr
     const double* y = L.xs;
     double best = 2.0, best_g = 0.0;
     uint32_t best_cls = 0, best_src = 0xFFFFFFFFu;
-    if (gs.has) {
-        const SourceDev* V = gs.S;
+    for_sources<G>(gs, [&](const SourceDev* V, uint32_t i) {
         auto local_at = [&](double ss, double l[3]) {
             to_local_v(V, dense_at(qx, y[0], ss), dense_at(qy, y[1], ss), dense_at(qz, y[2], ss), l);
         };
@@ -1260,7 +1262,7 @@ __device__ __forceinline__ uint32_t mlocate_event(const TraceParams& P, const Gr
             if (sh < best) {
                 best = sh;
                 best_cls = RT_CLS_CAPTURED;
-                best_src = si;
+                best_src = i;
             }
         }
         const double z0 = l0[2], z1 = l1[2];
@@ -1285,11 +1287,11 @@ __device__ __forceinline__ uint32_t mlocate_event(const TraceParams& P, const Gr
                 const double lzv = (l[0] * pl1 - l[1] * pl0) / fmax(pm, 1e-30);
                 best = sd;
                 best_cls = RT_CLS_DISK;
-                best_src = si;
+                best_src = i;
                 best_g = disk_redshift(V->m, V->a, rc, lzv, P.clamp_g != 0);
             }
         }
-    }
+    });
     {
         uint32_t win = best_src;
         group_argmin<G>(best, win);
@@ -1354,11 +1356,10 @@ __device__ __forceinline__ bool mstep(const TraceParams& P, const GroupSrc& gs, 
     if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
         L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
         double near = 0.0;
-        if (gs.has) {
-            const SourceDev* V = gs.S;
+        for_sources<G>(gs, [&](const SourceDev* V, uint32_t) {
             const double dx = L.xs[0] - V->c[0], dy = L.xs[1] - V->c[1], dz = L.xs[2] - V->c[2];
             if (sqrt(dx * dx + dy * dy + dz * dz) < 6.0 * V->m) near = 1.0;
-        }
+        });
         L.cls = (group_min<G>(-near) < 0.0) ? RT_CLS_CAPTURED : RT_CLS_ESCAPED;
         return true;
     }
@@ -1397,15 +1398,14 @@ __device__ __forceinline__ bool mstep(const TraceParams& P, const GroupSrc& gs, 
     // any event in this step? (a horizon reached or a disk plane crossed by one
     // of the group's sources, escape)
     double mine = 0.0;
-    if (gs.has) {
-        const SourceDev* V = gs.S;
+    for_sources<G>(gs, [&](const SourceDev* V, uint32_t) {
         double l0[3], l1[3];
         to_local_v(V, L.xs[0], L.xs[1], L.xs[2], l0);
         to_local_v(V, X5[0], X5[1], X5[2], l1);
         const bool hz = ks_radius(l1[0], l1[1], l1[2], V->a2) <= V->r_hz;
         const bool dk = V->disk_on && ((l0[2] < 0.0 && l1[2] >= 0.0) || (l0[2] > 0.0 && l1[2] <= 0.0));
         if (hz || dk) mine = 1.0;
-    }
+    });
     const bool src_ev = group_min<G>(-mine) < 0.0;
     const double R1 = sqrt(X5[0] * X5[0] + X5[1] * X5[1] + X5[2] * X5[2]);
     bool es = false;
@@ -1486,8 +1486,7 @@ __device__ __noinline__ void store_multi_rest(const OutPtrs& O, uint64_t idx, ui
     O.rejected[idx] = rej;
 }
 
-// Store: every owner its components (for G = 16 the lower half), lane 0 of
-// the group the rest, with BL coordinates taken in the frame of source 0 (the
+// Store: every owner its components, lane 0 of the group the rest, with BL coordinates taken in the frame of source 0 (the
 // central hole).
 template <int G>
 __device__ __forceinline__ void store_mlane(const OutPtrs& O, uint64_t idx, const MLane<MG<G>::C>& L,
@@ -1505,8 +1504,8 @@ __device__ __forceinline__ void store_mlane(const OutPtrs& O, uint64_t idx, cons
     if (l == 0) store_multi_rest(O, idx, L.cls, L.xs[0], L.xs[1], L.xs[2], L.lambda, L.g, L.acc, L.rej, T.src[0]);
 }
 
-// Superposition request: a persistent kernel, G lanes per ray (G = 2, 4, 8
-// or 16, lane s of a group takes source s and a share of the components).
+// Superposition request: a persistent kernel, G lanes per ray (G = 2, 4 or
+// 8; lane s of a group takes its sources and a share of the components).
 // Idle groups are refilled from a device counter (ballot over the group
 // leaders, one atomicAdd per refill, rank by popcount), so a group whose ray
 // terminates takes the next ray while the slow rays of the wave continue.
@@ -1532,7 +1531,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
     const uint32_t si = lane_id & (G - 1);
     const uint32_t lane0 = lane_id & ~static_cast<uint32_t>(G - 1);
     const uint64_t below = (lane0 == 0) ? 0ull : (~0ull >> (64 - lane0));
-    const GroupSrc gs{&sm.src[si < T.n ? si : 0], si < T.n};
+    const GroupSrc gs{sm.src, si, T.n};
     const MCtx cx{si & static_cast<uint32_t>(MG<G>::Gc - 1), static_cast<int>(4 * lane0)};
     const uint64_t total = (P.mode == 0) ? static_cast<uint64_t>(P.tiles_x) * ((P.n_rays / P.cols + 7) / 8) * 64
                                          : static_cast<uint64_t>(P.n_rays);
@@ -1669,7 +1668,7 @@ hipError_t wave_slots(uint32_t* slots) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &a, reinterpret_cast<const void*>(&k_trace_camera_queue<false, false>), 64, 0);
     if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(&k_trace_multi<16, 2>),
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(&k_trace_multi<8, 2>),
                                                          64, 0);
     if (e != hipSuccess) return e;
     *slots = static_cast<uint32_t>(cus) * static_cast<uint32_t>(std::max(1, std::min(a, b)));
@@ -1693,11 +1692,16 @@ hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const Launc
     return P.mixed ? launch_rays_t<true, false>(P, O, L, stream) : launch_rays_t<false, false>(P, O, L, stream);
 }
 
-// Lanes per superposition ray: the smallest power of two not below the
-// number of sources (one source per lane).
+// Lanes per superposition ray. Up to eight sources: the smallest power of
+// two not below the number of sources (one source per lane; these frames are
+// bound by the latency of their slowest rays). From nine sources: 8 lanes
+// (two sources per lane, lane_sources), which halves the lanes per ray and
+// shares the stage arithmetic and the sums over the group between twice the
+// sources (these frames are bound by instruction issue): 16 holes on a 512^2
+// frame 55 -> 49 ms, while 8 holes on 4 lanes went from 27 to 31 ms.
 uint32_t multi_ray_lanes(uint32_t n_src) {
     uint32_t g = 2;
-    while (g < n_src && g < 16) g <<= 1;
+    while (g < n_src && g < 8) g <<= 1;
     return g;
 }
 
@@ -1739,7 +1743,6 @@ hipError_t launch_trace_multi(const TraceParams& P, const OutPtrs& O, const Mult
         case 2: return launch_multi_g<2>(P, O, T, next, L, stream, max_grid, limit, active);
         case 4: return launch_multi_g<4>(P, O, T, next, L, stream, max_grid, limit, active);
         case 8: return launch_multi_g<8>(P, O, T, next, L, stream, max_grid, limit, active);
-        case 16: return launch_multi_g<16>(P, O, T, next, L, stream, max_grid, limit, active);
         default: return hipErrorInvalidValue;
     }
 }
