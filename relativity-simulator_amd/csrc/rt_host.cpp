@@ -280,34 +280,56 @@ void fill_sources(const rt_metric* mt, const rt_termination* tm, MultiDev& T) {
 // frame of kerr_schild_initial_momentum (core/kerr_schild_null_integrator.hpp:149-177).
 // Superposition: flat spherical observer position and the frame of
 // ms_initial_momentum (render/software_compute_engine.hpp:2919-2956).
-// The rectangle of tiles around the hole's image: rays with impact
-// parameter up to 10 M (the shadow, at most ~7 M across for |a| < M, and the
-// rays that orbit near its edge), where the rays that take the most steps
-// are. Flat-space estimate from the pinhole mapping of camera_outward: the
-// hole lies along -n_obs; a direction with components (f, r, u) on
-// (forward, right, up) lands at u = r / (f tan_half), v = u / (f tan_half).
-static void hot_tiles(TraceParams& P, double r_obs) {
-    const double n_obs[3] = {P.sth * P.cph, P.sth * P.sph, P.cth};
-    double cf = 0.0, cr = 0.0, cu = 0.0;
-    for (int i = 0; i < 3; ++i) {
-        cf -= P.fwd[i] * n_obs[i];
-        cr -= P.right[i] * n_obs[i];
-        cu -= P.up[i] * n_obs[i];
-    }
-    if (!(cf > 1e-6) || !(P.tan_half > 0.0)) return;  // hole behind the camera
+// The rectangle of tiles around the holes' images: rays with impact
+// parameter up to 10 m about a hole of mass m (the shadow, at most ~7 m
+// across for |a| < m, and the rays that orbit near its edge), where the rays
+// that take the most steps are; for a superposition the rectangle holds
+// every source's circle. Flat-space estimate from the pinhole mapping of
+// camera_outward: a direction with components (f, r, u) on (forward, right,
+// up) lands at u = r / (f tan_half), v = u / (f tan_half). A hole behind the
+// camera or off the traced window adds nothing.
+struct HotHole {
+    double c[3];  // world centre
+    double m;
+};
+
+static void hot_tiles(TraceParams& P, double r_obs, const HotHole* holes, uint32_t n_holes) {
+    const double X_obs[3] = {r_obs * P.sth * P.cph, r_obs * P.sth * P.sph, r_obs * P.cth};
     const double W = P.width, H = P.height;
-    const double u = cr / (cf * P.tan_half), v = cu / (cf * P.tan_half);
-    const double px = (u / P.aspect + 1.0) * 0.5 * W - 0.5 - static_cast<double>(P.col_first);
-    const double py = (1.0 - v) * 0.5 * H - 0.5 - static_cast<double>(P.row_first);
-    const double s = std::min(1.0, 10.0 * P.M / r_obs);
-    const double alpha = std::asin(s);
-    const double r_pix = (alpha >= 1.5) ? 1e9 : std::tan(alpha) / P.tan_half * 0.5 * H + 8.0;
     const double tiles_y = static_cast<double>((P.n_rays / P.cols + 7) / 8);
-    const double x0 = std::max(0.0, std::floor((px - r_pix) / 8.0));
-    const double x1 = std::min(static_cast<double>(P.tiles_x) - 1.0, std::floor((px + r_pix) / 8.0));
-    const double y0 = std::max(0.0, std::floor((py - r_pix) / 8.0));
-    const double y1 = std::min(tiles_y - 1.0, std::floor((py + r_pix) / 8.0));
-    if (!(x1 >= x0) || !(y1 >= y0)) return;  // image off the traced window
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    for (uint32_t k = 0; k < n_holes; ++k) {
+        double d[3], dist2 = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            d[i] = holes[k].c[i] - X_obs[i];
+            dist2 += d[i] * d[i];
+        }
+        const double dist = std::sqrt(dist2);
+        if (!(dist > 0.0) || !(P.tan_half > 0.0)) continue;
+        double cf = 0.0, cr = 0.0, cu = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            cf += P.fwd[i] * d[i] / dist;
+            cr += P.right[i] * d[i] / dist;
+            cu += P.up[i] * d[i] / dist;
+        }
+        if (!(cf > 1e-6)) continue;  // behind the camera
+        const double u = cr / (cf * P.tan_half), v = cu / (cf * P.tan_half);
+        const double px = (u / P.aspect + 1.0) * 0.5 * W - 0.5 - static_cast<double>(P.col_first);
+        const double py = (1.0 - v) * 0.5 * H - 0.5 - static_cast<double>(P.row_first);
+        const double s = std::min(1.0, 10.0 * holes[k].m / dist);
+        const double alpha = std::asin(s);
+        const double r_pix = (alpha >= 1.5) ? 1e9 : std::tan(alpha) / P.tan_half * 0.5 * H + 8.0;
+        const double hx0 = std::max(0.0, std::floor((px - r_pix) / 8.0));
+        const double hx1 = std::min(static_cast<double>(P.tiles_x) - 1.0, std::floor((px + r_pix) / 8.0));
+        const double hy0 = std::max(0.0, std::floor((py - r_pix) / 8.0));
+        const double hy1 = std::min(tiles_y - 1.0, std::floor((py + r_pix) / 8.0));
+        if (!(hx1 >= hx0) || !(hy1 >= hy0)) continue;  // image off the traced window
+        x0 = std::min(x0, hx0);
+        x1 = std::max(x1, hx1);
+        y0 = std::min(y0, hy0);
+        y1 = std::max(y1, hy1);
+    }
+    if (!(x1 >= x0) || !(y1 >= y0)) return;
     P.hot_x0 = static_cast<uint32_t>(x0);
     P.hot_y0 = static_cast<uint32_t>(y0);
     P.hot_w = static_cast<uint32_t>(x1 - x0 + 1.0);
@@ -372,7 +394,20 @@ void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T, const rt_tile*
     }
     P.hot_x0 = P.hot_y0 = P.hot_w = P.hot_h = 0;
     // hot tiles from the pinhole mapping; other projections trace row by row
-    if (P.n_src < 2 && (c->projection == RT_PROJ_PINHOLE || c->projection == RT_PROJ_AUTO_ZOOM)) hot_tiles(P, r_obs);
+    if (c->projection == RT_PROJ_PINHOLE || c->projection == RT_PROJ_AUTO_ZOOM) {
+        HotHole holes[kMaxSources];
+        uint32_t n_holes = 0;
+        if (P.n_src >= 2 && T) {
+            for (uint32_t i = 0; i < T->n && i < static_cast<uint32_t>(kMaxSources); ++i) {
+                for (int j = 0; j < 3; ++j) holes[n_holes].c[j] = T->src[i].c[j];
+                holes[n_holes++].m = T->src[i].m;
+            }
+        } else if (P.n_src < 2) {
+            holes[0] = HotHole{{0.0, 0.0, 0.0}, P.M};
+            n_holes = 1;
+        }
+        hot_tiles(P, r_obs, holes, n_holes);
+    }
     if (P.n_src >= 2 && T) {
         P.X0[0] = r_obs * P.sth * P.cph;
         P.X0[1] = r_obs * P.sth * P.sph;

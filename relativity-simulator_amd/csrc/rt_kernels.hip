@@ -1510,7 +1510,9 @@ __device__ __forceinline__ void store_mlane(const OutPtrs& O, uint64_t idx, cons
 // leaders, one atomicAdd per refill, rank by popcount), so a group whose ray
 // terminates takes the next ray while the slow rays of the wave continue.
 // Camera rays are handed out in 8x8-tile order (ray g: tile g / 64, pixel
-// g % 64). The source table is staged into LDS. *next starts at zero.
+// g % 64), the tiles around the holes' images first (tile_xy): the rays that
+// take the most steps start at once instead of forming the frame's tail. The
+// source table is staged into LDS. *next starts at zero.
 template <int G, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPtrs O,
                                                             const MultiDev* __restrict__ Tg,
