@@ -52,7 +52,7 @@ hipError_t launch_trace_render(const TraceParams& P, const OutPtrs& O, const Lau
 // Explicit initial states (O.in_x / O.in_p), single Kerr.
 hipError_t launch_trace_rays(const TraceParams& P, const OutPtrs& O, const LaunchShape& L, hipStream_t stream);
 // Superposition (camera or explicit rays per P.mode), table T in device memory.
-// Lanes per ray of the superposition kernel (one source per lane).
+// Lanes per ray of the superposition kernel (2, 4 or 8; one or two sources per lane).
 uint32_t multi_ray_lanes(uint32_t n_src);
 // max_grid > 0: at most that many workgroups (the serving loop's grant of
 // wave slots); 0: fill the device once. limit/active (serving loop): waves

@@ -780,8 +780,9 @@ __device__ __forceinline__ void group_argmin(double& s, uint32_t& i) {
 // Layout constants of a group of G lanes.
 template <int G>
 struct MG {
-    static constexpr int Gc = G < 8 ? G : 8;  // lanes owning distinct components
-    static constexpr int C = 8 / Gc;          // components per lane
+    static_assert(G == 2 || G == 4 || G == 8, "groups of 2, 4 or 8 lanes");
+    static constexpr int Gc = G;      // lanes owning components
+    static constexpr int C = 8 / G;   // components per lane
 };
 
 // Per-lane constants of the group layout.
@@ -813,7 +814,6 @@ __device__ __forceinline__ double from_owner(const double* v, const MCtx& cx) {
 template <int G>
 __device__ __forceinline__ void reduce_scatter(double v[8], uint32_t q, double* S) {
     constexpr int Gc = MG<G>::Gc;
-    static_assert(G == 2 || G == 4 || G == 8, "groups of 2, 4 or 8 lanes");
     if constexpr (Gc == 8) {
         // lanes 0-3 keep components 0-3, lanes 4-7 components 4-7 (partner 7 - q)
         const bool lo = q < 4;
@@ -1218,7 +1218,7 @@ __device__ __forceinline__ Dense dense_from_owner(const double* D, const double*
 // class (0: no event), the same in every lane of the group.
 template <int G>
 __device__ __forceinline__ uint32_t mlocate_event(const TraceParams& P, const GroupSrc& gs, const MCtx& cx,
-                                                  uint32_t si, uint32_t lane0, const MLane<MG<G>::C>& L, double h,
+                                                  uint32_t lane0, const MLane<MG<G>::C>& L, double h,
                                                   bool es, double* ye, double Xe[3], double Pe[3], double* s_out,
                                                   double* g_out) {
     constexpr int C = MG<G>::C;
@@ -1350,8 +1350,8 @@ __device__ __forceinline__ uint32_t mlocate_event(const TraceParams& P, const Gr
 }
 
 template <int G>
-__device__ __forceinline__ bool mstep(const TraceParams& P, const GroupSrc& gs, const MCtx& cx, uint32_t si,
-                                      uint32_t lane0, MLane<MG<G>::C>& L) {
+__device__ __forceinline__ bool mstep(const TraceParams& P, const GroupSrc& gs, const MCtx& cx, uint32_t lane0,
+                                      MLane<MG<G>::C>& L) {
     constexpr int Gc = MG<G>::Gc, C = MG<G>::C;
     if (L.acc + L.rej >= P.max_steps || L.ah < P.h_min) {
         L.flags |= (L.acc + L.rej >= P.max_steps) ? RT_FLAG_BUDGET : RT_FLAG_MIN_STEP;
@@ -1415,7 +1415,7 @@ __device__ __forceinline__ bool mstep(const TraceParams& P, const GroupSrc& gs, 
     }
     if (src_ev || es) {
         double s_ev = 1.0, g_ev = 0.0, ye[C], Xe[3], Pe[3];
-        const uint32_t c = mlocate_event<G>(P, gs, cx, si, lane0, L, h, es, ye, Xe, Pe, &s_ev, &g_ev);
+        const uint32_t c = mlocate_event<G>(P, gs, cx, lane0, L, h, es, ye, Xe, Pe, &s_ev, &g_ev);
         if (c != 0) {
             L.rhs += 6;
             if (mproject<G>(gs, Xe, Pe, L.pt, P.inv_tol)) {
@@ -1605,7 +1605,7 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
             if (drained) break;
             continue;
         }
-        if (active && mstep<G>(P, gs, cx, si, lane0, L)) {
+        if (active && mstep<G>(P, gs, cx, lane0, L)) {
             mlane_finish(L);
             store_mlane<G>(O, idx, L, T, cx, si);
             if (si == 0) tally.add(L);
