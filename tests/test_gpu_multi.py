@@ -68,7 +68,10 @@ def test_two_holes_camera(ctx):
     compare(gpu, ref, "two holes")
 
 
-@pytest.mark.parametrize("n", [4, 16])
+# Every lane-group shape of the kernel: 2 lanes (2 holes, above), 4 lanes with
+# an idle lane (3) and full (4), 8 lanes with idle lanes (5) and full (8),
+# 8 lanes with a second source on one lane (9) and on every lane (16).
+@pytest.mark.parametrize("n", [3, 4, 5, 8, 9, 16])
 def test_many_holes_camera(ctx, n):
     cam = rtsim.default_camera(300.0, math.radians(55.0), -0.4, math.radians(20.0), 24, 24)
     met = many_holes(n, 7 + n)
