@@ -502,9 +502,9 @@ rt_status rt_trace_batch(rt_context* ctx, const rt_request* reqs, size_t n_req, 
  * that does not fit is passed by later ones (until it has waited 20 ms, when
  * memory is what it lacks). Each admitted request runs at once, next to those
  * already on the device, as a persistent kernel sized to the wave slots it was
- * granted: single-Kerr requests up to their need, superpositions at most half
- * the device and never the last quarter, so that requests arriving meanwhile
- * find room. The kernel writes the outputs asked for straight into pinned
+ * granted: single-Kerr requests up to their need, superpositions at most three
+ * quarters of the device and never the last quarter, so that requests
+ * arriving meanwhile find room. The kernel writes the outputs asked for straight into pinned
  * staging. Every request must be taken (rt_server_take) or cancelled to free
  * its staging. */
 typedef struct rt_server_config {

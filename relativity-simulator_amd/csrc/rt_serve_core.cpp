@@ -210,7 +210,7 @@ uint32_t Core::take_slots(uint32_t need, bool multi) {
     const uint32_t free = cfg_.slots - slots_used_;
     uint32_t avail = free;
     if (multi) {
-        const uint32_t share = cfg_.multi_share ? cfg_.multi_share : std::max<uint32_t>(1, cfg_.slots / 2);
+        const uint32_t share = cfg_.multi_share ? cfg_.multi_share : std::max<uint32_t>(1, cfg_.slots * 3 / 4);
         const uint32_t reserve = cfg_.multi_reserve ? cfg_.multi_reserve : cfg_.slots / 4;
         avail = (free > reserve) ? std::min(free - reserve, share) : 0;
     }

@@ -338,10 +338,11 @@ static void shutdown_wakes_waiters() {
 }
 
 // Wave slots: grants never exceed the free slots; a single-Kerr request takes
-// the free slots up to its need; a superposition at most half of all slots
-// and never the last quarter; a request without room waits while later ones
-// that fit pass it; slots come back when the kernel ends; a cancel asks the
-// running kernel to stop (Resize to 0) but keeps its slots until it ends.
+// the free slots up to its need; a superposition at most three quarters of
+// all slots and never the last quarter; a request without room waits while
+// later ones that fit pass it; slots come back when the kernel ends; a cancel
+// asks the running kernel to stop (Resize to 0) but keeps its slots until it
+// ends.
 static Job mjob(uint32_t need) {
     Job j = sjob(need);
     j.multi = true;
@@ -369,22 +370,27 @@ static void wave_slots() {
     c.submit(sjob(30), &k1);
     c.submit(sjob(1), &k2);
     adm();
-    // m1: half of all slots; m2: what is free above the quarter kept back; m3: nothing
-    // left above it; k1: the free slots (the kept quarter); k2: none left
-    CHECK(ad.size() == 3 && ad[0].id == m1 && ad[0].slots == 50 && ad[1].id == m2 && ad[1].slots == 25);
-    CHECK(ad[2].id == k1 && ad[2].slots == 25 && c.counts().slots_free == 0);
-    CHECK(state(c, m3) == State::Pending && state(c, k2) == State::Pending && rs.empty());
+    // m1: three quarters of all slots; m2, m3: nothing free above the quarter
+    // kept back; k1: the free slots (the kept quarter); k2: none left
+    CHECK(ad.size() == 2 && ad[0].id == m1 && ad[0].slots == 75 && ad[1].id == k1 && ad[1].slots == 25);
+    CHECK(c.counts().slots_free == 0);
+    CHECK(state(c, m2) == State::Pending && state(c, m3) == State::Pending && state(c, k2) == State::Pending);
+    CHECK(rs.empty());
     c.finished(k1, RT_OK);
     adm();
-    CHECK(ad.size() == 1 && ad[0].id == k2 && ad[0].slots == 1);  // m3 still finds no room
+    CHECK(ad.size() == 1 && ad[0].id == k2 && ad[0].slots == 1);  // m2, m3 still find no room
     c.finished(m1, RT_OK);
     adm();
-    CHECK(ad.size() == 1 && ad[0].id == m3 && ad[0].slots == 49);  // 74 free: 49 above the quarter
+    CHECK(ad.size() == 1 && ad[0].id == m2 && ad[0].slots == 74);  // 99 free: 74 above the quarter
+    CHECK(state(c, m3) == State::Pending);
     c.cancel(m2);
     adm();
     CHECK(rs.size() == 1 && rs[0].id == m2 && rs[0].slots == 0);
-    CHECK(c.counts().slots_free == 100 - 25 - 49 - 1);  // its slots stay until its kernel ends
+    CHECK(c.counts().slots_free == 100 - 74 - 1);  // its slots stay until its kernel ends
+    CHECK(ad.empty() && state(c, m3) == State::Pending);
     c.finished(m2, RT_OK);
+    adm();
+    CHECK(ad.size() == 1 && ad[0].id == m3 && ad[0].slots == 74);
     c.finished(m3, RT_OK);
     c.finished(k2, RT_OK);
     CHECK(c.counts().slots_free == 100 && c.lanes_busy() == 0);
