@@ -3,9 +3,10 @@
 replayed.
 
 Each request's results must equal the single-request kernel's results for the
-same request (class identical, coordinates within 1e-12 relative: the same
-per-lane arithmetic runs in both kernels), on the first (captured) call and on
-every replay, with eager work between the replays."""
+same request (class, flags and step counts identical; coordinates within 1e-12
+relative: both kernels evaluate the same formulas, which the compiler may
+contract into FMAs differently, so the last bits can differ), on the first
+(captured) call and on every replay, with eager work between the replays."""
 import math
 
 import numpy as np
