@@ -422,4 +422,58 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+
+// Lane-internal flag bit (not exported): the event test found a sign change
+// in the last accepted attempt but no event inside it (a crossing of z = 0
+// off the disk); the attempt is repeated and taken as a plain step.
+constexpr uint32_t kNoEventRetry = 1u << 29;
+
+// Per-wave totals of the step counters, added to the launch's counters once
+// per wave (counters: [0] accepted [1] rejected [2] rhs evals [3] fp32 steps
+// [4] projections).
+struct Tally {
+    unsigned long long acc = 0, rej = 0, rhs = 0, f32 = 0, proj = 0;
+    template <class LaneT>
+    __device__ void add(const LaneT& L) {
+        acc += L.acc;
+        rej += L.rej;
+        rhs += L.rhs;
+        f32 += L.f32;
+        proj += L.proj;
+    }
+    __device__ void flush(unsigned long long* counters) {
+        const unsigned long long sa = wave_sum(acc), sr = wave_sum(rej), sh = wave_sum(rhs), sf = wave_sum(f32),
+                                 sp = wave_sum(proj);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&counters[0], sa);
+            atomicAdd(&counters[1], sr);
+            atomicAdd(&counters[2], sh);
+            atomicAdd(&counters[3], sf);
+            atomicAdd(&counters[4], sp);
+        }
+    }
+};
+
+// Wave-level yield of the serving loop's persistent kernels. *active counts
+// the request's running waves; a wave leaves when more are running than the
+// host's current limit for the request (a word in mapped pinned host memory,
+// read at system scope, lowered by the host to make room for other requests).
+// The host keeps the limit at 1 or more, so the request's last waves always
+// drain its queue. limit == nullptr: never yield.
+__device__ __forceinline__ bool wave_yields(const uint32_t* limit, unsigned long long* active) {
+    if (limit == nullptr) return false;
+    int go = 0;
+    if ((threadIdx.x & 63) == 0) {
+        const uint32_t lim = __hip_atomic_load(limit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const unsigned long long a = __hip_atomic_load(active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a > lim && atomicCAS(active, a, a - 1) == a) go = 1;
+    }
+    return __shfl(go, 0, 64) != 0;
+}
+
+__device__ __forceinline__ void wave_joins(const uint32_t* limit, unsigned long long* active) {
+    if (limit != nullptr && (threadIdx.x & 63) == 0) atomicAdd(active, 1ull);
+}
+
+
 }  // namespace rt

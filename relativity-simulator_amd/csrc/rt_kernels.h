@@ -43,6 +43,12 @@ hipError_t launch_trace_camera(const TraceParams& P, const OutPtrs& O, const Lau
 hipError_t launch_trace_camera_queue(const TraceParams& P, const OutPtrs& O, unsigned long long* next, uint32_t grid,
                                      hipStream_t stream, const uint32_t* limit = nullptr,
                                      unsigned long long* active = nullptr);
+// Block sizes the kernels accept: one to four waves.
+inline bool shape_ok(const LaunchShape& L) { return L.block != 0 && L.block <= 256 && (L.block & 63) == 0; }
+
+// Workgroups of the superposition kernel resident on one compute unit (rt_multi.hip).
+hipError_t multi_blocks_per_cu(int* blocks);
+
 // Wave slots of the device for the serving loop's kernels: one-wave
 // workgroups of the persistent camera and superposition kernels that can be
 // resident at once (compute units x workgroups per compute unit).
