@@ -8,7 +8,10 @@ generated on the device and results stay in HBM: the four frames are one
 batch (rt_trace_batch, one persistent kernel, replayed from a hipGraph);
 --path plain traces them with one launch per frame (rt_trace_camera_resident).
 --workload uneven512 times the 512-request batched workload instead.
-The "tiles" object reports the 32x32 tile-task path on the a = 0.9 frame.
+The "tiles" object reports the 32x32 tile-task path on the a = 0.9 frame;
+"render" the renderer's own RK4 tracer (RT_METHOD_RENDER_RK4) on a 2048^2
+near-observer disk scene, with the CPU restatement of that tracer and the
+reference's dispatch_fp64 timed on row bands of the same frame.
 
 Prints one JSON line. value = rays/s over the timed steps. roofline: the
 algorithmic FP64 FLOP per launch (right-hand-side evaluations x FLOP_RHS +
@@ -304,6 +307,70 @@ def superposition_report(side: int = 512, steps: int = 3):
     return {"frames": f"{side}^2, 2/4/8/16 holes", "kernel": "k_trace_multi", "per_frame": rows}
 
 
+def render_request(side: int):
+    """The renderer's own tracer (RT_METHOD_RENDER_RK4) on a near-observer
+    disk scene at side x side: a = 0.9, observer at 30 M, inclination 65 deg,
+    vertical FOV 60 deg, translucent disk to 24 M, escape at 100 M, step
+    budget 2048 (the scene of the reference-record fixture disk_pinhole)."""
+    cam = rtsim.default_camera(30.0, math.radians(65.0), 0.0, math.radians(60.0), side, side)
+    tm = rtsim.default_termination(escape_radius=100.0, disk_outer=24.0, disk_enabled=True)
+    return cam, rtsim.kerr_metric(1.0, 0.9), rtsim.render_integrator(2048), tm
+
+
+def render_report(side: int, steps: int = 3, band: int = 16, band_stride: int = 128, threads: int = 0,
+                  cpu: bool = True):
+    """RT_METHOD_RENDER_RK4 frame on the GPU (k_trace_render, one launch,
+    median kernel time of `steps`), beside two CPU timings of bands of `band`
+    rows every `band_stride` rows of the same frame: the oracle's restatement
+    of the same tracer (records only, like the kernel), and, where
+    oracle/_ref/libref.so was built, the reference's dispatch_fp64 itself
+    (which also shades each pixel's colour)."""
+    cam, met, ig, tm = render_request(side)
+    ctx = rtsim.Context(0)
+    n = ctx.trace_camera_resident(cam, met, ig, tm)  # warm-up
+    ms, st = [], None
+    for _ in range(steps):
+        ctx.trace_camera_resident(cam, met, ig, tm)
+        st = ctx.stats()
+        ms.append(st.kernel_ms)
+    ctx.close()
+    t = float(np.median(ms))
+    out = {"frame": f"render_rk4_kerr_{side}sq_a0.9_i65_robs30_fov60_disk", "kernel": "k_trace_render",
+           "rays": n, "kernel_ms": t, "rays_per_s": n / (t * 1e-3), "mean_iterations": st.accepted_steps / n}
+    if not cpu:
+        return out
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib
+
+    cpus = host_cpus()
+    threads = threads or cpus["threads"]
+    native = native_oracle()
+    if native:
+        oracle_lib.ORACLE_PATH = native
+    firsts = list(range(0, side, band_stride))
+    sample = f"rows [k*{band_stride}, k*{band_stride}+{band}) of the frame ({len(firsts) * band * side} rays)"
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        for f in firsts:
+            cam.row_offset, cam.row_count = f, min(band, side - f)
+            fn()
+        cam.row_offset, cam.row_count = 0, 0
+        return time.perf_counter() - t0
+
+    nr = sum(min(band, side - f) for f in firsts) * side
+    dt = timed(lambda: oracle_lib.trace_render_camera(cam, met, ig, tm, threads=threads))
+    out["cpu_port"] = {"value": nr / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+                       "sample": f"{sample}, {dt:.2f} s, oracle/{os.path.basename(oracle_lib.ORACLE_PATH)}"}
+    if oracle_lib.ref_available():
+        dt = timed(lambda: oracle_lib.ref_render_frame(cam, met, ig, tm))
+        out["cpu_reference"] = {"value": nr / dt, "unit": "rays/s", "cores": os.cpu_count(), "kind": "reference",
+                                "sample": f"{sample}, {dt:.2f} s, SoftwareComputeEngine::dispatch_fp64 "
+                                          "(one thread per online CPU, records and colour) from "
+                                          "oracle/_ref/libref.so built -O2"}
+    return out
+
+
 def copy_probe(mib: int):
     """Three plain device-to-device copies of a known size: what FETCH_SIZE
     and WRITE_SIZE read for them calibrates the counters against bytes."""
@@ -337,6 +404,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tiles", action="store_true", help="skip the 32x32 tile-task report")
+    ap.add_argument("--no-render", action="store_true", help="skip the RT_METHOD_RENDER_RK4 report")
     ap.add_argument("--workload", choices=["frames4", "uneven512", "multi"], default="frames4",
                     help="frames4: the four 2048^2 frames; uneven512: 512 seeded camera requests in one batch; "
                          "multi: superposition frames of 2, 4, 8, 16 holes (512^2), one launch each")
@@ -462,6 +530,9 @@ def main() -> int:
     if args.workload == "frames4":
         print("bench: superposition frames", file=sys.stderr, flush=True)
         out["superposition"] = superposition_report()
+    if not args.no_render and args.workload == "frames4":
+        print("bench: renderer's RK4 tracer", file=sys.stderr, flush=True)
+        out["render"] = render_report(args.side, cpu=not args.no_cpu, threads=args.cpu_threads)
     if args.precision_side and args.workload == "frames4":
         print(f"bench: precision, mixed vs FP64 on {args.precision_side}^2", file=sys.stderr, flush=True)
         out["precision"] = precision_report(args.precision_side, args.switch_radius)
