@@ -307,25 +307,17 @@ def superposition_report(side: int = 512, steps: int = 3):
     return {"frames": f"{side}^2, 2/4/8/16 holes", "kernel": "k_trace_multi", "per_frame": rows}
 
 
-def render_request(side: int):
-    """The renderer's own tracer (RT_METHOD_RENDER_RK4) on a near-observer
-    disk scene at side x side: a = 0.9, observer at 30 M, inclination 65 deg,
-    vertical FOV 60 deg, translucent disk to 24 M, escape at 100 M, step
-    budget 2048 (the scene of the reference-record fixture disk_pinhole)."""
-    cam = rtsim.default_camera(30.0, math.radians(65.0), 0.0, math.radians(60.0), side, side)
-    tm = rtsim.default_termination(escape_radius=100.0, disk_outer=24.0, disk_enabled=True)
-    return cam, rtsim.kerr_metric(1.0, 0.9), rtsim.render_integrator(2048), tm
-
-
 def render_report(side: int, steps: int = 3, band: int = 16, band_stride: int = 128, threads: int = 0,
                   cpu: bool = True):
-    """RT_METHOD_RENDER_RK4 frame on the GPU (k_trace_render, one launch,
-    median kernel time of `steps`), beside two CPU timings of bands of `band`
-    rows every `band_stride` rows of the same frame: the oracle's restatement
+    """RT_METHOD_RENDER_RK4 frame on the GPU (workloads.render_request:
+    a = 0.9, observer at 30 M, inclination 65 deg, FOV 60 deg, disk to 24 M,
+    escape at 100 M; k_trace_render, one launch, median kernel time of
+    `steps`), beside two CPU timings of bands of `band` rows every
+    `band_stride` rows of the same frame: the oracle's restatement
     of the same tracer (records only, like the kernel), and, where
     oracle/_ref/libref.so was built, the reference's dispatch_fp64 itself
     (which also shades each pixel's colour)."""
-    cam, met, ig, tm = render_request(side)
+    cam, met, ig, tm = workloads.render_request(side)
     ctx = rtsim.Context(0)
     n = ctx.trace_camera_resident(cam, met, ig, tm)  # warm-up
     ms, st = [], None

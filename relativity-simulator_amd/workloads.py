@@ -1,4 +1,4 @@
-"""Seeded synthetic trace requests of the tracer's five workloads.
+"""Seeded synthetic trace requests of the tracer's workloads.
 
 Shared by bench.py and the GPU tests so that both trace the same rays.
 
@@ -13,6 +13,9 @@ Shared by bench.py and the GPU tests so that both trace the same rays.
                of 2..16 holes (masses U[0.1, 1], positions U[-25, 25]^3 M,
                spins U[0, 0.95] on random axes)
   multi        superposition frames of 2, 4, 8 and 16 holes drawn as in serving
+  render       the renderer's own tracer (RT_METHOD_RENDER_RK4) on a near
+               observer: a = 0.9, r_obs = 30 M, inclination 65 deg, FOV 60 deg,
+               translucent disk to 24 M, escape at 100 M, budget 2048 steps
 """
 from __future__ import annotations
 
@@ -100,3 +103,11 @@ def multi_requests(side: int = 512, counts: Tuple[int, ...] = (2, 4, 8, 16), see
         out.append(rtsim.make_request(cam, random_holes(rng, k), rtsim.default_integrator(),
                                       rtsim.default_termination()))
     return out
+
+
+def render_request(side: int = 2048):
+    """The render workload as (camera, metric, integrator, termination): the
+    scene of the reference-record fixture disk_pinhole at side x side."""
+    cam = rtsim.default_camera(30.0, math.radians(65.0), 0.0, math.radians(60.0), side, side)
+    tm = rtsim.default_termination(escape_radius=100.0, disk_outer=24.0, disk_enabled=True)
+    return cam, rtsim.kerr_metric(1.0, 0.9), rtsim.render_integrator(2048), tm

@@ -6,6 +6,9 @@
   the FP64 parity bar (test_gpu_parity.compare), and property checks on
   every ray (class set, g per class, step totals equal to the kernel's
   counters).
+- The renderer's tracer (RT_METHOD_RENDER_RK4) on the 2048^2 render
+  workload: every 32nd row against the oracle's restatement, records
+  identical and end states at the FP64 bar.
 - Mixed precision on Kerr a = 0.998, inclination 85 deg, 4096^2 with the FP64
   right-hand side inside 3 r_+: classes identical to the FP64 kernel on all
   16.8 M rays, per-output error against the FP64 kernel within the bars
@@ -97,6 +100,40 @@ def test_frames4_batch_vs_oracle(ctx):
         ref, rows = oracle_rows(r.camera, r.metric, r.integrator, r.termination, 32)
         compare(rows_of(o, 2048, rows), ref, f"2048^2 a={a} every 32nd row")
         assert {1, 2, 4} <= set(ref.klass.tolist())
+
+
+def test_render_workload_vs_oracle(ctx):
+    """The renderer's tracer on the full 2048^2 render workload: every 32nd
+    row against the oracle's restatement (records identical, end states at
+    the FP64 bar), and the step counter equal to the per-ray steps."""
+    cam, met, ig, tm = workloads.render_request(2048)
+    gpu = ctx.trace_camera(cam, met, ig, tm)
+    st = ctx.stats()
+    assert gpu.n == 2048 * 2048 and st.accepted_steps == int(gpu.steps.sum())
+    assert np.all(gpu.rejected == 0) and np.all(gpu.lam == gpu.steps * 0.05)
+    rows = list(range(0, 2048, 32))
+    parts = []
+    for r in rows:
+        cam.row_offset, cam.row_count = r, 1
+        parts.append(O.trace_render_camera(cam, met, ig, tm))
+    cam.row_offset, cam.row_count = 0, 0
+    sub = rows_of(gpu, 2048, rows)
+    same = {"status": 0, "iters": 0}
+    for i, ref in enumerate(parts):
+        s = slice(i * 2048, (i + 1) * 2048)
+        same["status"] += int((sub.cls[s] != ref.cls).sum())
+        same["iters"] += int((sub.steps[s] != ref.steps).sum())
+        for f in ["x", "p", "bl"]:
+            d = rel(getattr(sub, f)[:, s], getattr(ref, f)).max()
+            assert d < 1e-9, f"row {rows[i]}: {f} differs by {d:.2e}"
+        assert rel(sub.g[s], ref.g).max() < 1e-12, f"row {rows[i]}: g differs"
+    k = gpu.cls & 0xFF
+    print(f"[render 2048^2] rows {len(rows)}, mismatches {same}, mean steps {gpu.steps.mean():.2f}, "
+          f"disk bit {((k & rtsim.RT_CLS_DISK) != 0).mean():.3f}, "
+          f"captured {((k & rtsim.RT_CLS_CAPTURED) != 0).mean():.3f}",
+          flush=True)
+    assert same == {"status": 0, "iters": 0}
+    assert ((k & rtsim.RT_CLS_DISK) != 0).any() and ((k & rtsim.RT_CLS_CAPTURED) != 0).any()
 
 
 def mixed_errors(mx, ref):
