@@ -185,7 +185,7 @@ def precision_report(side: int, switch_radius: float):
     their ratio, per-output error (non-captured rays; max(|ref|, 1)
     denominator; phi as a wrapped angle), and class agreement overall and by
     distance to the critical curve (rt_critical_distance: rays within 1e-6 M
-    of it are the north star's exclusion band)."""
+    of it are exempt from the class-match requirement)."""
     cam, met, ig64, tm = workloads.precision_request(side)
     igmx = workloads.precision_request(side, rtsim.RT_PRECISION_MIXED, switch_radius)[2]
     ctx = rtsim.Context(0)
@@ -411,7 +411,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this tracer runs on one GPU; --gpus {args.gpus} is not supported "
-              "(rays shard with zero exchange, see DESIGN.md)", file=sys.stderr)
+              "(rays are independent: a frame shards by tiles across GPUs with no exchange)", file=sys.stderr)
         return 2
     if args.precision_report:
         print(json.dumps(precision_report(args.precision_report, args.switch_radius)), flush=True)
