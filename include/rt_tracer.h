@@ -226,8 +226,10 @@ typedef struct rt_camera {
                              angle; each projection reads it as the reference does) */
     uint32_t width;
     uint32_t height;
-    uint32_t row_offset;  /* first traced row (dispatch_row_offset)               */
-    uint32_t row_count;   /* traced rows; 0 = all (dispatch_row_count)            */
+    uint32_t row_offset;  /* first traced row (dispatch_row_offset); read only
+                             when row_count is not 0                           */
+    uint32_t row_count;   /* traced rows, clipped at the last row; 0 = the whole
+                             frame (dispatch_row_count)                        */
     uint32_t projection;  /* RT_PROJ_* (projection_mode); 0 = pinhole           */
     uint32_t reserved;
 } rt_camera;

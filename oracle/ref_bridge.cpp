@@ -406,8 +406,9 @@ void ref_render_frame(const rt_camera* cam, const rt_metric* mt, const rt_integr
     pc.primary_disk.enabled = tm->disk_enabled ? 1.0f : 0.0f;
     pc.primary_disk.inner_radius_scale = tm->disk_inner > 0.0 ? static_cast<float>(tm->disk_inner / isco) : 1.0f;
     pc.primary_disk.outer_radius_mass_units = static_cast<float>((tm->disk_outer > 0.0 ? tm->disk_outer : 20.0 * m) / m);
-    const uint32_t first = std::min(cam->row_offset, cam->height);
-    const uint32_t last = cam->row_count ? std::min(first + cam->row_count, cam->height) : cam->height;
+    const uint32_t first = cam->row_count ? std::min(cam->row_offset, cam->height) : 0u;
+    const uint32_t last =
+        cam->row_count ? (uint32_t)std::min<uint64_t>((uint64_t)first + cam->row_count, cam->height) : cam->height;
     const size_t n = static_cast<size_t>(last - first) * cam->width;
     std::vector<Render::GpuPixelOutput> px(n);
     Render::SoftwareComputeEngine::dispatch_fp64(pc, std::span<Render::GpuPixelOutput>(px.data(), n));

@@ -1099,13 +1099,26 @@ using namespace rto;
 
 extern "C" {
 
+
+// The camera's row window [first, last), as resolve_row_window computes it
+// (render/software_compute_engine.hpp:1042-1050): count 0 is the whole frame,
+// whatever the offset; otherwise offset + count in 64 bits, clipped.
+static void row_window(const rt_camera* cam, uint32_t& first, uint32_t& last) {
+    first = 0;
+    last = cam->height;
+    if (cam->row_count) {
+        first = std::min(cam->row_offset, cam->height);
+        last = (uint32_t)std::min<uint64_t>((uint64_t)first + cam->row_count, cam->height);
+    }
+}
+
 int rto_trace_camera(const rt_camera* cam, const rt_metric* mt, const rt_integrator* ig, const rt_termination* tm,
                      rt_results* out, int threads) {
     if (!cam || !mt || !ig || !tm || !out || cam->width == 0 || cam->height == 0) return RT_ERR_INVALID_ARG;
     const Spacetime st = make_spacetime(mt, tm);
     const Integ in = make_integ(ig);
-    const uint32_t first = std::min(cam->row_offset, cam->height);
-    const uint32_t last = cam->row_count ? std::min(first + cam->row_count, cam->height) : cam->height;
+    uint32_t first, last;
+    row_window(cam, first, last);
     const size_t W = cam->width;
     const size_t n = (size_t)(last - first) * W;
     parallel_rows(last - first, threads, [&](size_t row) {
@@ -1180,8 +1193,8 @@ int rto_trace_render_camera(const rt_camera* cam, const rt_metric* mt, const rt_
         return RT_ERR_INVALID_ARG;
     const Spacetime st = make_spacetime(mt, tm);
     const RenderCfg rc = make_render(cam, mt, ig, tm);
-    const uint32_t first = std::min(cam->row_offset, cam->height);
-    const uint32_t last = cam->row_count ? std::min(first + cam->row_count, cam->height) : cam->height;
+    uint32_t first, last;
+    row_window(cam, first, last);
     const size_t W = cam->width;
     const size_t n = (size_t)(last - first) * W;
     parallel_rows(last - first, threads, [&](size_t row) {

@@ -337,8 +337,14 @@ static void hot_tiles(TraceParams& P, double r_obs, const HotHole* holes, uint32
 }
 
 void fill_camera(const rt_camera* c, TraceParams& P, MultiDev* T, const rt_tile* win) {
-    uint32_t first = std::min(c->row_offset, c->height);
-    uint32_t last = c->row_count ? std::min(first + c->row_count, c->height) : c->height;
+    // the row window as resolve_row_window takes it (render/software_compute_engine.hpp:1042-1050):
+    // count 0 is the whole frame (the offset ignored); otherwise offset + count in 64 bits,
+    // clipped at the last row
+    uint32_t first = 0, last = c->height;
+    if (c->row_count) {
+        first = std::min(c->row_offset, c->height);
+        last = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(first) + c->row_count, c->height));
+    }
     uint32_t col0 = 0, cols = c->width;
     if (win) {  // a rectangle of the frame (checked by valid_window)
         first = win->origin_y;

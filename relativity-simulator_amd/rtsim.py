@@ -310,9 +310,12 @@ def make_request(cam: rt_camera, metric: rt_metric, integ: rt_integrator, term: 
 
 
 def n_rays_of(cam: rt_camera) -> int:
+    """Rays of a camera's row window: all rows when row_count is 0 (the
+    offset ignored), else rows [offset, offset + count) clipped at the last."""
+    if not cam.row_count:
+        return cam.height * cam.width
     first = min(cam.row_offset, cam.height)
-    last = min(first + cam.row_count, cam.height) if cam.row_count else cam.height
-    return (last - first) * cam.width
+    return (min(first + cam.row_count, cam.height) - first) * cam.width
 
 
 # ------------------------------------------------------------ library

@@ -1,5 +1,6 @@
 """Edge cases on the GPU: frames smaller than one 8x8 tile and ragged sizes,
-explicit-ray counts that do not fill a wave, requests the C ABI must refuse,
+explicit-ray counts that do not fill a wave, a row window whose count runs
+past the last row, requests the C ABI must refuse,
 a one-request batch, and a full-size frame of the precision configuration
 (Kerr a = 0.998, i = 85 deg, 4096^2) checked through properties that do not
 depend on its size plus every 64th row against the oracle."""
@@ -35,6 +36,30 @@ def test_ragged_ray_counts(ctx, n):
     compare(ctx.trace_rays(x, p, met, ig, tm), O.trace_rays(x, p, met, ig, tm), f"{n} rays")
 
 
+def test_row_count_past_the_last_row(ctx):
+    """row_offset + row_count beyond 2^32: the window ends at the last row (the
+    reference's 64-bit row window), alone and in a batch; row_count 0 traces
+    the whole frame whatever row_offset says."""
+    met, ig = rtsim.kerr_metric(1.0, 0.7), rtsim.default_integrator()
+    tm = rtsim.default_termination(escape_radius=100.0)
+    far = rtsim.default_camera(30.0, math.radians(70.0), 0.4, math.radians(40.0), 24, 20, row_offset=7,
+                               row_count=0xFFFFFFFF)
+    exact = rtsim.default_camera(30.0, math.radians(70.0), 0.4, math.radians(40.0), 24, 20, row_offset=7,
+                                 row_count=13)
+    a, b = ctx.trace_camera(far, met, ig, tm), ctx.trace_camera(exact, met, ig, tm)
+    assert a.n == b.n == 13 * 24
+    for f in ["cls", "steps", "x", "p", "bl", "g"]:
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    outs = ctx.trace_batch([rtsim.make_request(far, met, ig, tm)])
+    assert np.array_equal(outs[0].cls, b.cls) and np.array_equal(outs[0].x, b.x)
+    # count 0 is the whole frame whatever the offset (resolve_row_window)
+    whole = rtsim.default_camera(30.0, math.radians(70.0), 0.4, math.radians(40.0), 24, 20)
+    shifted = rtsim.default_camera(30.0, math.radians(70.0), 0.4, math.radians(40.0), 24, 20, row_offset=7)
+    c, d = ctx.trace_camera(shifted, met, ig, tm), ctx.trace_camera(whole, met, ig, tm)
+    assert c.n == d.n == 20 * 24 and np.array_equal(c.x, d.x) and np.array_equal(c.cls, d.cls)
+    assert np.array_equal(d.x[:, 7 * 24:], b.x)
+
+
 def test_invalid_requests_are_refused(ctx):
     met, ig, tm = rtsim.kerr_metric(1.0, 0.5), rtsim.default_integrator(), rtsim.default_termination()
     cam = rtsim.default_camera(100.0, 1.0, 0.0, 0.3, 8, 8)
@@ -43,7 +68,7 @@ def test_invalid_requests_are_refused(ctx):
     bad_cams.append(c)
     c = rtsim.default_camera(100.0, 1.0, 0.0, 0.0, 8, 8)  # zero field of view
     bad_cams.append(c)
-    c = rtsim.default_camera(100.0, 1.0, 0.0, 0.3, 8, 8, row_offset=8)  # empty row window
+    c = rtsim.default_camera(100.0, 1.0, 0.0, 0.3, 8, 8, row_offset=8, row_count=1)  # empty row window
     bad_cams.append(c)
     for c in bad_cams:
         with pytest.raises(rtsim.RtError):

@@ -160,3 +160,28 @@ def test_random_frames_against_reference():
         assert all(v == 0 for v in bad.values()), (k, bad)
         total += got["status"].size
     assert total > 2000
+
+
+@ref_only
+def test_row_windows_against_reference():
+    """Row windows through the oracle and the reference's dispatch_fp64
+    (resolve_row_window): count 0 with and without an offset, an inner window,
+    a count running past the last row, and offset + count beyond 2^32. The
+    reference writes nothing when its buffer is smaller than its window, so a
+    window of the wrong size shows up as mismatched records."""
+    met, ig = rtsim.kerr_metric(1.0, 0.8), rtsim.render_integrator(600)
+    tm = rtsim.default_termination(escape_radius=80.0, disk_outer=20.0)
+    whole = None
+    for off, cnt in [(0, 0), (5, 0), (3, 7), (14, 50), (10, 0xFFFFFFFF), (19, 1)]:
+        cam = rtsim.default_camera(35.0, 1.2, 0.3, 0.8, 16, 20, row_offset=off, row_count=cnt)
+        want = O.ref_render_frame(cam, met, ig, tm)
+        got = pixel_view(O.trace_render_camera(cam, met, ig, tm))
+        n_rows = 20 if cnt == 0 else min(off + cnt, 20) - off
+        assert got["status"].size == want["status"].size == n_rows * 16, (off, cnt)
+        bad = mismatches(got, want)
+        assert all(v == 0 for v in bad.values()), (off, cnt, bad)
+        if (off, cnt) == (0, 0):
+            whole = want
+        else:
+            first = 0 if cnt == 0 else off
+            assert np.array_equal(want["iters"], whole["iters"][first * 16:(first + n_rows) * 16]), (off, cnt)

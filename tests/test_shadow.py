@@ -89,3 +89,28 @@ def test_distance_marks_the_class_boundary():
     al, be = rtsim.shadow_boundary(1.0, a, theta, 4000)
     captured_area = (cls == rtsim.RT_CLS_CAPTURED).sum() * pix * pix
     assert captured_area == pytest.approx(polygon_area(al, be), rel=0.03)
+
+
+def test_row_window_count_to_the_last_row():
+    """A row count that runs past the last row (up to 2^32 - 1) means "to the
+    last row", as the reference's row window takes offset + count in 64 bits:
+    the same distances as the exact count, and the oracle's frame the same.
+    A count of 0 is the whole frame, whatever the offset."""
+    met = rtsim.kerr_metric(1.0, 0.9)
+    for count in (0xFFFFFFFF, 0xFFFFFFF0, 100):
+        far = rtsim.default_camera(30.0, 1.1, 0.2, 0.9, 12, 20, row_offset=5, row_count=count)
+        exact = rtsim.default_camera(30.0, 1.1, 0.2, 0.9, 12, 20, row_offset=5, row_count=15)
+        assert rtsim.n_rays_of(far) == 15 * 12
+        assert np.array_equal(rtsim.critical_distance(far, met), rtsim.critical_distance(exact, met))
+        ig, tm = rtsim.default_integrator(), rtsim.default_termination(escape_radius=100.0)
+        a, b = O.trace_camera(far, met, ig, tm), O.trace_camera(exact, met, ig, tm)
+        assert np.array_equal(a.cls, b.cls) and np.array_equal(a.x, b.x)
+        ra = O.trace_render_camera(far, met, rtsim.render_integrator(), tm)
+        rb = O.trace_render_camera(exact, met, rtsim.render_integrator(), tm)
+        assert np.array_equal(ra.cls, rb.cls) and np.array_equal(ra.steps, rb.steps)
+    # count 0: the whole frame, the offset ignored
+    whole = rtsim.default_camera(30.0, 1.1, 0.2, 0.9, 12, 20)
+    shifted = rtsim.default_camera(30.0, 1.1, 0.2, 0.9, 12, 20, row_offset=5)
+    assert rtsim.n_rays_of(shifted) == 20 * 12
+    dw, ds = rtsim.critical_distance(whole, met), rtsim.critical_distance(shifted, met)
+    assert np.array_equal(dw, ds) and np.array_equal(dw[5 * 12:], rtsim.critical_distance(exact, met))
