@@ -943,6 +943,14 @@ __global__ __launch_bounds__(256, Waves) void k_trace_multi(TraceParams P, OutPt
             if (drained) break;
             continue;
         }
+        // A wave holding a ray past 300 attempts issues first on its SIMD: the
+        // frame ends with those rays, and a wave beside them on the SIMD would
+        // otherwise take half the issue slots while the frame's bulk runs
+        // (8 and 16 holes on 512^2: 23.3 -> 22.2 and 47.4 -> 43.9 ms).
+        if (__ballot(active && (L.acc + L.rej) > 300u) != 0)
+            __builtin_amdgcn_s_setprio(3);
+        else
+            __builtin_amdgcn_s_setprio(0);
         if (active && mstep<G>(P, gs, cx, lane0, L)) {
             mlane_finish(L);
             store_mlane<G>(O, idx, L, T, cx, si);
