@@ -101,15 +101,21 @@ def host_cpus():
     return info
 
 
+_NATIVE = []
+
+
 def native_oracle():
     """Build the oracle for this host (-O3 -march=native, as the reference's
-    own build flags) and return its path; None if the build fails."""
+    own build flags) once per process and return its path; None if the build
+    fails."""
     import subprocess
 
-    r = subprocess.run(["make", "-B", "-s", "-C", os.path.join(REPO, "oracle"), "liboracle_native.so"],
-                       capture_output=True, text=True)
-    path = os.path.join(REPO, "oracle", "liboracle_native.so")
-    return path if r.returncode == 0 and os.path.exists(path) else None
+    if not _NATIVE:
+        r = subprocess.run(["make", "-B", "-s", "-C", os.path.join(REPO, "oracle"), "liboracle_native.so"],
+                           capture_output=True, text=True)
+        path = os.path.join(REPO, "oracle", "liboracle_native.so")
+        _NATIVE.append(path if r.returncode == 0 and os.path.exists(path) else None)
+    return _NATIVE[0]
 
 
 def cpu_baseline(side: int, stride: int, threads: int):
