@@ -9,6 +9,8 @@
 - The renderer's tracer (RT_METHOD_RENDER_RK4) on the 2048^2 render
   workload: every 32nd row against the oracle's restatement, records
   identical and end states at the FP64 bar.
+- The superposition frames (512^2, 2/4/8/16 holes): every 32nd row against
+  the oracle at the superposition bar.
 - Mixed precision on Kerr a = 0.998, inclination 85 deg, 4096^2 with the FP64
   right-hand side inside 3 r_+: classes identical to the FP64 kernel on all
   16.8 M rays, per-output error against the FP64 kernel within the bars
@@ -134,6 +136,21 @@ def test_render_workload_vs_oracle(ctx):
           flush=True)
     assert same == {"status": 0, "iters": 0}
     assert ((k & rtsim.RT_CLS_DISK) != 0).any() and ((k & rtsim.RT_CLS_CAPTURED) != 0).any()
+
+
+def test_superposition_frames_vs_oracle(ctx):
+    """The bench's superposition frames (512^2 with 2, 4, 8 and 16 holes),
+    each one k_trace_multi launch: every 32nd row against the oracle at the
+    superposition bar, and the step counters equal to the per-ray steps."""
+    for r in workloads.multi_requests(512):
+        k = r.metric.n_sources
+        gpu = ctx.trace_camera(r.camera, r.metric, r.integrator, r.termination)
+        st = ctx.stats()
+        assert gpu.n == 512 * 512
+        assert st.accepted_steps == int(gpu.steps.sum()) and st.rejected_steps == int(gpu.rejected.sum())
+        assert set(np.unique(gpu.klass).tolist()) <= {1, 2, 4}
+        ref, rows = oracle_rows(r.camera, r.metric, r.integrator, r.termination, 32)
+        compare_multi(rows_of(gpu, 512, rows), ref, f"512^2 {k} holes every 32nd row")
 
 
 def mixed_errors(mx, ref):
