@@ -313,19 +313,21 @@ def superposition_report(side: int = 512, steps: int = 3):
     return {"frames": f"{side}^2, 2/4/8/16 holes", "kernel": "k_trace_multi", "per_frame": rows}
 
 
-def render_report(side: int, steps: int = 3, band: int = 16, band_stride: int = 128, threads: int = 0,
-                  cpu: bool = True):
+def render_report(side: int, steps: int = 5, warmup: int = 5, band: int = 16, band_stride: int = 128,
+                  threads: int = 0, cpu: bool = True):
     """RT_METHOD_RENDER_RK4 frame on the GPU (workloads.render_request:
     a = 0.9, observer at 30 M, inclination 65 deg, FOV 60 deg, disk to 24 M,
     escape at 100 M; k_trace_render, one launch, median kernel time of
-    `steps`), beside two CPU timings of bands of `band` rows every
+    `steps` after `warmup` launches: a ~3 ms launch keeps getting faster over
+    its first few repeats), beside two CPU timings of bands of `band` rows every
     `band_stride` rows of the same frame: the oracle's restatement
     of the same tracer (records only, like the kernel), and, where
     oracle/_ref/libref.so was built, the reference's dispatch_fp64 itself
     (which also shades each pixel's colour)."""
     cam, met, ig, tm = workloads.render_request(side)
     ctx = rtsim.Context(0)
-    n = ctx.trace_camera_resident(cam, met, ig, tm)  # warm-up
+    for _ in range(warmup):
+        n = ctx.trace_camera_resident(cam, met, ig, tm)
     ms, st = [], None
     for _ in range(steps):
         ctx.trace_camera_resident(cam, met, ig, tm)
