@@ -287,7 +287,7 @@ def launch_flops(st, n_src: int) -> float:
     return st.rhs_evals * per_rhs + (st.accepted_steps + st.rejected_steps) * FLOP_ATTEMPT
 
 
-def superposition_report(side: int = 512, steps: int = 3):
+def superposition_report(side: int = 512, steps: int = 5):
     """Superposition frames of 2, 4, 8 and 16 holes (workloads.multi_requests:
     masses U[0.1, 1], positions U[-25, 25]^3 M, spins U[0, 0.95] on random
     axes; observer 1000 M, inclination 60 deg, FOV 20 deg), each traced by
