@@ -5,7 +5,8 @@ Every request's result must be the same alone (single-request kernel) and in
 the mix of concurrently served requests (class identical, coordinates within
 1e-12 relative); a request too large for the budget is refused with
 RT_ERR_BUSY; a cancelled request reports RT_ERR_CANCELLED and one past its
-deadline RT_ERR_EXPIRED; only the outputs asked for come back. The request
+deadline RT_ERR_EXPIRED; only the outputs asked for come back; requests of
+the renderer's RK4 method are served with identical records. The request
 states, admission and deadlines themselves are checked on the CPU with an
 injected clock (tests/test_serve_core.py)."""
 import math
@@ -107,3 +108,29 @@ def test_deadline_cancel_and_shutdown_with_work_on_the_device(ctx):
         assert np.array_equal(got.cls, alone.cls) and np.array_equal(got.x, alone.x)
         for _ in range(3):
             srv.submit(big)  # left on the device: the server's exit waits for them
+
+
+def test_render_requests_served(ctx):
+    """RT_METHOD_RENDER_RK4 requests served beside DP45 ones: records and end
+    states identical to the same request traced alone (the DP45 ones within
+    1e-12, as above: the serving loop traces them with its queue kernel)."""
+    def render_req(side, a, inc, phi):
+        cam = rtsim.default_camera(30.0, math.radians(inc), phi, math.radians(60.0), side, side + 7)
+        tm = rtsim.default_termination(escape_radius=100.0, disk_outer=24.0)
+        return rtsim.make_request(cam, rtsim.kerr_metric(1.0, a), rtsim.render_integrator(), tm)
+
+    reqs = [render_req(40, 0.9, 65, 0.3), req(48, 0.9, 60), render_req(56, -0.6, 80, -1.0), req(24, 0.5, 30)]
+    alone = [ctx.trace_camera(r.camera, r.metric, r.integrator, r.termination) for r in reqs]
+    with rtsim.Server(0, hbm_budget_bytes=64 << 20, max_inflight=8) as srv:
+        ids = [srv.submit(r) for r in reqs]
+        for i, rid in enumerate(ids):
+            srv.wait(rid, 60000)
+            got = srv.take(rid)
+            assert np.array_equal(got.cls, alone[i].cls) and np.array_equal(got.steps, alone[i].steps), i
+            for name in ("x", "p", "bl", "lam", "g"):
+                u, v = getattr(got, name), getattr(alone[i], name)
+                if reqs[i].integrator.method == rtsim.RT_METHOD_RENDER_RK4:
+                    assert np.array_equal(u, v), f"request {i}: {name} differs"  # the same kernel
+                else:
+                    d = (np.abs(u - v) / np.maximum(np.abs(v), 1.0)).max(initial=0)
+                    assert d < 1e-12, f"request {i}: {name} differs by {d:.3e}"
