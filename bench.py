@@ -61,10 +61,12 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per ray of the tracing kernels on MI355X: rocprofv3 FETCH_SIZE +
 # WRITE_SIZE per dispatch, each scaled by what the counter reads for a plain
 # device-to-device copy of known size (x2.000 and x1.000), over the rays of
-# the dispatch (4 x 2048^2 Kerr frames). A camera ray reads nothing and
-# writes 116 B of results.
+# the dispatch (4 x 2048^2 Kerr frames; for the superposition kernel the four
+# 512^2 frames of 2/4/8/16 holes, 2077/1536/2450/2344 B, averaged). A camera
+# ray reads nothing and writes 116 B of results; the superposition kernel's
+# excess is its per-lane scratch reaching HBM.
 HBM_BYTES_PER_RAY = {"k_trace_camera_kerr": 119.86, "k_trace_tiles": 116.55, "k_trace_queue": 116.55,
-                     "k_trace_multi": None}
+                     "k_trace_multi": 2101.8}
 METRIC = "rays/s Kerr 2048\u00b2 mixed vs FP64 at \u22641e-9 rel err; p50/p99 latency @QPS"
 
 
